@@ -1,0 +1,175 @@
+/*
+ * ouxsbli.h -- C ABI of the MI355X (gfx950) RHS + Runge-Kutta library.
+ *
+ * The reference solver is a monolithic CUDA-Fortran executable: its hot path is
+ * `RungeKutta` (3D_solver/src/calc_time_dev.f90.fypp:30, :224) driving
+ * `calc_EFG` (3D_solver/src/calc_flux_base.f90.fypp:109), the `calc_step*`
+ * kernels (3D_solver/src/calc_steps.f90:42-184) and `set_bc`
+ * (3D_solver/NSTGV/set.f90:50-57). It has no library boundary; the Python layer
+ * (ouxsbli/case.py:180-201) shells out to `mpirun a.out`. Every entry point
+ * below replaces one piece of that executable, as cited on each declaration.
+ *
+ * Conventions
+ *   - Every function returns OX_OK (0) on success and a negative OX_E* code on
+ *     failure; ox_last_error() returns a thread-local message for the last
+ *     failure on the calling thread. Nothing here calls exit() or abort().
+ *   - Host pointers are borrowed for the duration of the call only (copies are
+ *     synchronous). The handle owns all device memory, its stream and graphs.
+ *   - A handle is not thread-safe; distinct handles may be driven from
+ *     distinct threads.
+ *   - State arrays use the reference layout Q(nx,5,ny,nz) in Fortran order
+ *     (x fastest, then the variable, then y, then z), ghost layers included:
+ *     element (i,l,j,k) (1-based) lives at
+ *     (i-1) + nx*((l-1) + 5*((j-1) + ny*(k-1))).
+ *     Variables are (rho, rho*u, rho*v, rho*w, E) -- the conserved set.
+ */
+#ifndef OUXSBLI_H
+#define OUXSBLI_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OX_ABI_VERSION 1
+
+/* Error codes. */
+#define OX_OK            0
+#define OX_EINVAL       -1   /* bad argument / configuration              */
+#define OX_ENOMEM       -2   /* device or host allocation failed          */
+#define OX_EDEVICE      -3   /* HIP runtime error                         */
+#define OX_EUNSUPPORTED -4   /* configuration not built into the library  */
+#define OX_ESTATE       -5   /* non-physical state (rho<=0, p<=0, NaN)    */
+#define OX_EBUSY        -6   /* admission refused: footprint over budget  */
+#define OX_ENOTFOUND    -7   /* unknown case id                           */
+
+/* config.fypp SCHEME (src/mod_constant.f90.fypp:85-93) */
+#define OX_SCHEME_KEEP   0
+#define OX_SCHEME_SLAU   1
+#define OX_SCHEME_ROE    2
+#define OX_SCHEME_HYBRID 3
+/* config.fypp VISC (src/mod_constant.f90.fypp:37-43) */
+#define OX_VISC_EULER 0
+#define OX_VISC_NS    1
+#define OX_VISC_LES   2
+/* config.fypp TVD (src/mod_constant.f90.fypp:51-57) */
+#define OX_TVD_NONE   0
+#define OX_TVD_TVD    1
+#define OX_TVD_HYBRID 2
+/* config.fypp SLAU_VARIANT (src/mod_constant.f90.fypp:58-62) */
+#define OX_SLAU_SLAU1   0
+#define OX_SLAU_HRSLAU2 1
+/* VISC_STENCIL (3D_solver/src/calc_visc_high_internal.f90.fypp:5-7) */
+#define OX_STENCIL_CENTRAL 0
+#define OX_STENCIL_ME4BASE 1
+/* Arithmetic of the RHS. FP64: everything FP64 (bit-identical to the FP64
+ * CPU transcription). FP32_BULK: flux/stencil arithmetic in FP32, conserved
+ * state, RK accumulation and primitive recovery in FP64. */
+#define OX_PREC_FP64      0
+#define OX_PREC_FP32_BULK 1
+
+/* Compile-time switches of one case (config.fypp) plus the physical
+ * constants of mod_globals.f90 that the hot path reads.  Replaces the
+ * fypp/`parameter` specialisation of the reference
+ * (src/mod_constant.f90.fypp:14-100, 3D_solver/NSTGV/mod_globals.f90). */
+typedef struct ox_config {
+  int32_t abi_version;   /* must be OX_ABI_VERSION                              */
+  int32_t nx, ny, nz;    /* extents incl. order/2 ghost layers on each side     */
+  int32_t order;         /* ORDER: 2, 4 or 6                                    */
+  int32_t scheme;        /* OX_SCHEME_*                                         */
+  int32_t visc;          /* OX_VISC_*                                           */
+  int32_t visc_order;    /* VISC_ORDER: 4 or 6 (0 = same as order)              */
+  int32_t visc_stencil;  /* OX_STENCIL_*                                        */
+  int32_t tvd;           /* OX_TVD_*                                            */
+  int32_t slau_variant;  /* OX_SLAU_*                                           */
+  int32_t rk;            /* RK: 3 (TVD, Shu-Osher) or 4 (low-storage)           */
+  int32_t precision;     /* OX_PREC_*                                           */
+  int32_t bc_x, bc_y, bc_z; /* 0 = periodic; non-periodic is not supported    */
+  int32_t device;        /* HIP device ordinal                                  */
+  double gamma, R, Pr, Prt;
+  double threshold;      /* hybrid switch on the Ducros sensor                  */
+  double dt;             /* fixed time step (mod_globals.f90 `dt`)             */
+  double Lx, Ly, Lz;     /* periodic box lengths                                */
+} ox_config;
+
+/* Interior diagnostics, FP64 with compensated summation.  ke and entropy are
+ * the quantities of src/print.f90:42-122 (interior means); enstrophy is the
+ * mean of 0.5*rho*|curl u|^2 (2nd-order central differences);
+ * max_wave_speed = max(|u|+c, |v|+c, |w|+c) for CFL. */
+typedef struct ox_diag {
+  double ke;          /* mean 0.5*rho*|u|^2                      */
+  double entropy;     /* mean rho*ln(p*rho^-gamma)               */
+  double enstrophy;   /* mean 0.5*rho*|omega|^2                  */
+  double max_wave_speed;
+  double mass;        /* sum rho*vol                             */
+  double energy;      /* sum E*vol                               */
+} ox_diag;
+
+typedef struct ox_solver ox_solver;
+
+/* Creates a solver for one case: allocates the device state, the grid metrics
+ * and scratch.  Replaces allocate_device_mem + pre_calc's metric setup
+ * (3D_solver/src/preprocess.f90.fypp:21-46, :48-103) and set_grid_cyclic /
+ * set_Jacobian_xy3 (src/set_coordinate.f90:99-117, :197-339). */
+int ox_create(const ox_config* cfg, ox_solver** out);
+
+/* Frees everything the handle owns.  Replaces the deallocate at
+ * 3D_solver/src/calc_time_dev.f90.fypp:110. */
+void ox_destroy(ox_solver* s);
+
+/* Uploads the physical conserved state Q(nx,5,ny,nz) and divides by the
+ * Jacobian (QJ = Q / J), as pre_calc does before its H2D copy
+ * (3D_solver/src/preprocess.f90.fypp:65-71, :95). */
+int ox_set_state(ox_solver* s, const double* Q);
+
+/* Downloads J*QJ, the physical conserved state written to recal/Q*.dat at the
+ * end of the run (3D_solver/src/main.f90:82-93). */
+int ox_get_state(ox_solver* s, double* Q);
+
+/* Raw volume-scaled state QJ(nx,5,ny,nz) (the device array QJ of
+ * calc_time_dev.f90.fypp:47), without the Jacobian multiply/divide. */
+int ox_set_qj(ox_solver* s, const double* QJ);
+int ox_get_qj(ox_solver* s, double* QJ);
+
+/* Advances nsteps full RK steps (RK3: 3 stages, RK4: 4 stages) with the
+ * periodic ghost fill after every stage.  Replaces the inner `t1` loop of
+ * RungeKutta (3D_solver/src/calc_time_dev.f90.fypp:84-101 and :266-286). */
+int ox_step(ox_solver* s, int32_t nsteps);
+
+/* One right-hand-side evaluation on the handle's current state: writes
+ * R(nx,5,ny,nz) = dt*A*(div of face fluxes) at interior cells (zero in the
+ * ghost layers), i.e. calc_EFG followed by calc_R
+ * (3D_solver/src/calc_flux_base.f90.fypp:109-198, calc_steps.f90:8-38).
+ * Parity hook; not used by ox_step. */
+int ox_rhs(ox_solver* s, double* R);
+
+/* Interior diagnostics of the current state on the device.  Replaces the
+ * host-side FP32 print_KE / print_entropy (src/print.f90:42-122). */
+int ox_diagnostics(ox_solver* s, ox_diag* out);
+
+/* Scans the current state for non-physical cells (rho<=0, p<=0, non-finite).
+ * Returns OX_OK, or OX_ESTATE with *bad_index set to the first offending
+ * linear cell index (x fastest over nx,ny,nz) and *count to how many.
+ * The reference has no such watchdog (preprocess.f90.fypp:143-148 checks one
+ * rescaling array only). */
+int ox_check_state(ox_solver* s, int64_t* bad_index, int64_t* count);
+
+/* Blocks until all work queued on the handle's stream has finished. */
+int ox_synchronize(ox_solver* s);
+
+/* Device time of the last ox_step call in milliseconds (HIP events on the
+ * handle's stream). */
+int ox_last_step_ms(ox_solver* s, double* ms);
+
+/* Thread-local text of the last error on this thread ("" if none). */
+const char* ox_last_error(void);
+
+/* Library version string ("ouxsbli-mi355x <abi> <arch>"). */
+const char* ox_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OUXSBLI_H */

@@ -1,0 +1,91 @@
+"""ctypes mirror of ``include/ouxsbli.h`` (the C ABI of libouxsbli).
+
+Only layout and constants live here; the loader is ``_native.py``.
+"""
+import ctypes
+
+OX_ABI_VERSION = 1
+
+OX_OK = 0
+OX_EINVAL = -1
+OX_ENOMEM = -2
+OX_EDEVICE = -3
+OX_EUNSUPPORTED = -4
+OX_ESTATE = -5
+OX_EBUSY = -6
+OX_ENOTFOUND = -7
+
+SCHEME = {"KEEP": 0, "SLAU": 1, "Roe": 2, "Hybrid": 3}
+VISC = {"Euler": 0, "NS": 1, "LES": 2}
+TVD = {"none": 0, "tvd": 1, "hybrid": 2}
+SLAU_VARIANT = {"SLAU": 0, "HRSLAU2": 1}
+STENCIL = {"Central": 0, "ME4Base": 1}
+PRECISION = {"fp64": 0, "fp32_bulk": 1}
+
+
+class OxConfig(ctypes.Structure):
+    """``ox_config`` -- one case's compile-time switches and constants."""
+
+    _fields_ = [
+        ("abi_version", ctypes.c_int32),
+        ("nx", ctypes.c_int32),
+        ("ny", ctypes.c_int32),
+        ("nz", ctypes.c_int32),
+        ("order", ctypes.c_int32),
+        ("scheme", ctypes.c_int32),
+        ("visc", ctypes.c_int32),
+        ("visc_order", ctypes.c_int32),
+        ("visc_stencil", ctypes.c_int32),
+        ("tvd", ctypes.c_int32),
+        ("slau_variant", ctypes.c_int32),
+        ("rk", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("bc_x", ctypes.c_int32),
+        ("bc_y", ctypes.c_int32),
+        ("bc_z", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("gamma", ctypes.c_double),
+        ("R", ctypes.c_double),
+        ("Pr", ctypes.c_double),
+        ("Prt", ctypes.c_double),
+        ("threshold", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("Lx", ctypes.c_double),
+        ("Ly", ctypes.c_double),
+        ("Lz", ctypes.c_double),
+    ]
+
+
+class OxDiag(ctypes.Structure):
+    """``ox_diag`` -- interior diagnostics."""
+
+    _fields_ = [
+        ("ke", ctypes.c_double),
+        ("entropy", ctypes.c_double),
+        ("enstrophy", ctypes.c_double),
+        ("max_wave_speed", ctypes.c_double),
+        ("mass", ctypes.c_double),
+        ("energy", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# Every symbol include/ouxsbli.h declares (checked by tests/test_abi.py).
+EXPORTS = (
+    "ox_create",
+    "ox_destroy",
+    "ox_set_state",
+    "ox_get_state",
+    "ox_set_qj",
+    "ox_get_qj",
+    "ox_step",
+    "ox_rhs",
+    "ox_diagnostics",
+    "ox_check_state",
+    "ox_synchronize",
+    "ox_last_step_ms",
+    "ox_last_error",
+    "ox_version",
+)

@@ -1,0 +1,208 @@
+"""Case specifications: grids, physical constants and initial conditions.
+
+The reference encodes each case as a Fortran ``module set`` plus
+``mod_globals.f90`` parameters and builds one executable per case.  Here a case
+is a :class:`CaseSpec` (the ``ox_config`` switches plus the physical constants
+the initial condition needs) and a numpy initial-condition generator that
+returns the physical conserved state ``Q(nx,5,ny,nz)`` (Fortran order, ghosts
+filled periodically) ready for ``ox_set_state``.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Optional
+
+import numpy as np
+
+from . import _abi
+
+# Thermodynamic constants shared by the 3D cases (3D_solver/NSTGV/mod_globals.f90:39-43).
+GAMMA = 1.4
+R_GAS = 287.03
+PR = 0.71
+PRT = 0.9
+T_NSTGV = 530.0 * 5.0 / 9.0  # NSTGV/mod_globals.f90:47
+SUTHERLAND_S = 111.0
+
+
+def sutherland_mu(T: float) -> float:
+    """mu0 of NSTGV/mod_globals.f90:49 (Sutherland's law, S = 111 K)."""
+    return 1.716e-5 * (273.2 + SUTHERLAND_S) / (T + SUTHERLAND_S) * (T / 273.2) ** 1.5
+
+
+@dataclasses.dataclass
+class CaseSpec:
+    """One solve: extents incl. ghosts, scheme switches and constants."""
+
+    nx: int
+    ny: int
+    nz: int
+    order: int = 4
+    scheme: str = "KEEP"
+    visc: str = "NS"
+    visc_order: int = 0
+    visc_stencil: str = "Central"
+    tvd: str = "hybrid"
+    slau_variant: str = "HRSLAU2"
+    rk: int = 4
+    precision: str = "fp64"
+    gamma: float = GAMMA
+    R: float = R_GAS
+    Pr: float = PR
+    Prt: float = PRT
+    threshold: float = 0.4
+    dt: float = 1e-3
+    Lx: float = 2.0 * math.pi
+    Ly: float = 2.0 * math.pi
+    Lz: float = 2.0 * math.pi
+    device: int = 0
+    # initial-condition constants (not part of ox_config)
+    ic: str = "tgv"
+    L0: float = 1.0
+    V0: float = 1.0
+    RHO0: float = 1.0
+    p0: float = 1.0
+    T0: float = T_NSTGV
+    seed: Optional[int] = None
+
+    @property
+    def ghost(self) -> int:
+        return self.order // 2
+
+    @property
+    def interior(self):
+        g = self.ghost
+        return (self.nx - 2 * g, self.ny - 2 * g, self.nz - 2 * g)
+
+    @property
+    def interior_cells(self) -> int:
+        a, b, c = self.interior
+        return a * b * c
+
+    @property
+    def stages(self) -> int:
+        return 3 if self.rk == 3 else 4
+
+    def to_ox(self) -> _abi.OxConfig:
+        c = _abi.OxConfig()
+        c.abi_version = _abi.OX_ABI_VERSION
+        c.nx, c.ny, c.nz = self.nx, self.ny, self.nz
+        c.order = self.order
+        c.scheme = _abi.SCHEME[self.scheme]
+        c.visc = _abi.VISC[self.visc]
+        c.visc_order = self.visc_order
+        c.visc_stencil = _abi.STENCIL[self.visc_stencil]
+        c.tvd = _abi.TVD[self.tvd]
+        c.slau_variant = _abi.SLAU_VARIANT[self.slau_variant]
+        c.rk = self.rk
+        c.precision = _abi.PRECISION[self.precision]
+        c.bc_x = c.bc_y = c.bc_z = 0
+        c.device = self.device
+        c.gamma, c.R, c.Pr, c.Prt = self.gamma, self.R, self.Pr, self.Prt
+        c.threshold = self.threshold
+        c.dt = self.dt
+        c.Lx, c.Ly, c.Lz = self.Lx, self.Ly, self.Lz
+        return c
+
+    def state_shape(self):
+        """numpy shape of Q(nx,5,ny,nz) in C order (z, y, var, x)."""
+        return (self.nz, self.ny, 5, self.nx)
+
+
+def cell_centres(n: int, L: float, order: int) -> np.ndarray:
+    """Cell centres xc(1..n) of set_grid_cyclic{2,4,6}_3D (src/set_coordinate.f90:120-339).
+
+    Node i sits at dx1*(i-1-order/2) with dx1 = L/(n-order); ghost nodes extend
+    the same spacing; centres are midpoints.
+    """
+    off = order // 2
+    dx1 = L / float(n - order)
+    idx = np.arange(1, n + 2, dtype=np.float64)
+    x = dx1 * (idx - 1 - off)
+    return 0.5 * (x[:-1] + x[1:])
+
+
+def periodic_fill(Q: np.ndarray, order: int) -> np.ndarray:
+    """Ghost layers = periodic image of the interior (set_bc_cyclic*_init,
+    3D_solver/src/set_bc_common.f90:11-97).  Q has shape (nz, ny, 5, nx)."""
+    h = order // 2
+    nz, ny, _, nx = Q.shape
+    ix, iy, iz = nx - 2 * h, ny - 2 * h, nz - 2 * h
+    Q[:, :, :, :h] = Q[:, :, :, ix:ix + h]
+    Q[:, :, :, nx - h:] = Q[:, :, :, h:2 * h]
+    Q[:, :h, :, :] = Q[:, iy:iy + h, :, :]
+    Q[:, ny - h:, :, :] = Q[:, h:2 * h, :, :]
+    Q[:h, :, :, :] = Q[iz:iz + h, :, :, :]
+    Q[nz - h:, :, :, :] = Q[h:2 * h, :, :, :]
+    return Q
+
+
+def tgv_spec(n: int, order: int, Ma: float, Re: float = 1600.0, cfl: float = 0.3, L0: float = 1.0,
+             cfl_mode: str = "reference", **kw) -> CaseSpec:
+    """Taylor-Green vortex on [0, 2*pi*L0)^3 (3D_solver/NSTGV/mod_globals.f90:10-58).
+
+    ``n`` is the interior size per side; extents add ``order/2`` ghosts per side.
+    cfl_mode "reference": dt = CFL * (Lx/(nx-1)) / V0, the NSTGV formula
+    (mod_globals.f90:54; nx counts ghosts), a convective Courant number.
+    cfl_mode "acoustic": dt = CFL * dx / (V0 + c0) with dx = Lx/n, the Courant
+    number of the fastest wave.  At Ma 0.1 the reference formula with CFL 0.3
+    puts the acoustic Courant number at 3.2, beyond RK4's stability limit.
+    """
+    N = n + order
+    T0 = T_NSTGV
+    mu0 = sutherland_mu(T0)
+    V0 = Ma * math.sqrt(GAMMA * R_GAS * T0)
+    RHO0 = mu0 * Re / (V0 * L0)
+    p0 = RHO0 * R_GAS * T0
+    L = 2.0 * math.pi * L0
+    if cfl_mode == "reference":
+        dt = cfl * (L / float(N - 1)) / V0
+    elif cfl_mode == "acoustic":
+        c0 = math.sqrt(GAMMA * R_GAS * T0)
+        dt = cfl * (L / float(n)) / (V0 + c0)
+    else:
+        raise ValueError(cfl_mode)
+    spec = CaseSpec(nx=N, ny=N, nz=N, order=order, dt=dt, Lx=L, Ly=L, Lz=L, ic="tgv", L0=L0, V0=V0,
+                    RHO0=RHO0, p0=p0, T0=T0)
+    for k, v in kw.items():
+        setattr(spec, k, v)
+    return spec
+
+
+def tgv_state(spec: CaseSpec) -> np.ndarray:
+    """NSTGV set_init (3D_solver/NSTGV/set.f90:16-47): conserved Q at interior
+    cell centres, then the periodic ghost fill.  Returns shape (nz, ny, 5, nx)."""
+    L0, V0, RHO0, p0, T = spec.L0, spec.V0, spec.RHO0, spec.p0, spec.T0
+    x = cell_centres(spec.nx, spec.Lx, spec.order)
+    y = cell_centres(spec.ny, spec.Ly, spec.order)
+    z = cell_centres(spec.nz, spec.Lz, spec.order)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    p = p0 + RHO0 * (V0 ** 2) * (np.cos(2.0 * X / L0) + np.cos(2.0 * Y / L0)) * (np.cos(2.0 * Z / L0) + 2.0) / 16.0
+    rho = p / (spec.R * T)
+    Q = np.zeros(spec.state_shape(), dtype=np.float64)
+    Q[:, :, 0, :] = rho
+    Q[:, :, 1, :] = rho * V0 * np.sin(X / L0) * np.cos(Y / L0) * np.cos(Z / L0)
+    Q[:, :, 2, :] = -rho * V0 * np.cos(X / L0) * np.sin(Y / L0) * np.cos(Z / L0)
+    Q[:, :, 3, :] = 0.0
+    Q[:, :, 4, :] = p / (spec.gamma - 1.0) + 0.5 * (Q[:, :, 1, :] ** 2 + Q[:, :, 2, :] ** 2) / Q[:, :, 0, :]
+    return periodic_fill(Q, spec.order)
+
+
+def initial_state(spec: CaseSpec) -> np.ndarray:
+    if spec.ic == "tgv":
+        return tgv_state(spec)
+    raise ValueError(f"unknown initial condition {spec.ic!r}")
+
+
+# The benchmark workloads (interior sizes; extents add the ghosts).
+def config1_spec(n: int = 64) -> CaseSpec:
+    """Plain FP64 NS TGV, KEEP ORDER 4, central-4 viscous, RK4, Ma 0.1, acoustic CFL 0.3."""
+    return tgv_spec(n, 4, Ma=0.1, cfl=0.3, cfl_mode="acoustic", scheme="KEEP", visc="NS", rk=4, tvd="none")
+
+
+def config2_spec(n: int = 256, precision: str = "fp64") -> CaseSpec:
+    """Supersonic NS TGV, Ma 1.25, SLAU ORDER 6 HR-SLAU2, hybrid TVD (threshold 0.4),
+    central-6 viscous, RK3-TVD, CFL 0.03 (the NSTGV case file)."""
+    return tgv_spec(n, 6, Ma=1.25, cfl=0.03, scheme="SLAU", visc="NS", rk=3, tvd="hybrid",
+                    slau_variant="HRSLAU2", threshold=0.4, precision=precision)

@@ -1,0 +1,112 @@
+"""ctypes loader for the ORACLE (oracle/liboxoracle.so).
+
+Test infrastructure: the oracle is the checker the HIP library is compared
+against.  Builds the library on first use if it is missing (make -C oracle).
+"""
+import ctypes
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+
+from ouxsbli_amd import _abi
+
+ROOT = pathlib.Path(__file__).resolve().parent.parent
+LIB = ROOT / "oracle" / "liboxoracle.so"
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            subprocess.run(["make", "-C", str(ROOT / "oracle")], check=True, capture_output=True)
+        L = ctypes.CDLL(str(LIB))
+        P = ctypes.c_void_p
+        D = ctypes.POINTER(ctypes.c_double)
+        L.oxo_create.restype = P
+        L.oxo_create.argtypes = [ctypes.POINTER(_abi.OxConfig)]
+        L.oxo_destroy.argtypes = [P]
+        for f in ("oxo_set_state", "oxo_get_state", "oxo_set_qj", "oxo_get_qj", "oxo_rhs"):
+            getattr(L, f).argtypes = [P, D]
+            getattr(L, f).restype = ctypes.c_int
+        L.oxo_step.argtypes = [P, ctypes.c_int]
+        L.oxo_step.restype = ctypes.c_int
+        L.oxo_get_primitives.argtypes = [P, D, D, D, D]
+        L.oxo_diagnostics.argtypes = [P, ctypes.POINTER(_abi.OxDiag)]
+        L.oxo_last_error.restype = ctypes.c_char_p
+        L.oxo_keep_flux.argtypes = [ctypes.POINTER(_abi.OxConfig), ctypes.c_int, D, ctypes.c_int, D]
+        L.oxo_slau_flux.argtypes = [ctypes.POINTER(_abi.OxConfig), ctypes.c_int, D, ctypes.c_int,
+                                    ctypes.c_double, D]
+        L.oxo_muscl.argtypes = [ctypes.POINTER(_abi.OxConfig), ctypes.c_int, ctypes.c_int, ctypes.c_double, D,
+                                D, D]
+        L.oxo_wiggle.argtypes = [D]
+        L.oxo_wiggle.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Oracle:
+    """The CPU transcription driven through its C entry points."""
+
+    def __init__(self, spec):
+        self.spec = spec
+        self.cfg = spec.to_ox()
+        self.h = lib().oxo_create(ctypes.byref(self.cfg))
+        if not self.h:
+            raise RuntimeError(lib().oxo_last_error().decode())
+
+    def close(self):
+        if self.h:
+            lib().oxo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_state(self, Q):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        lib().oxo_set_state(self.h, _dp(Q))
+
+    def get_state(self):
+        Q = np.empty(self.spec.state_shape())
+        lib().oxo_get_state(self.h, _dp(Q))
+        return Q
+
+    def set_qj(self, QJ):
+        QJ = np.ascontiguousarray(QJ, dtype=np.float64)
+        lib().oxo_set_qj(self.h, _dp(QJ))
+
+    def get_qj(self):
+        QJ = np.empty(self.spec.state_shape())
+        lib().oxo_get_qj(self.h, _dp(QJ))
+        return QJ
+
+    def step(self, n=1):
+        lib().oxo_step(self.h, int(n))
+
+    def rhs(self):
+        R = np.empty(self.spec.state_shape())
+        lib().oxo_rhs(self.h, _dp(R))
+        return R
+
+    def primitives(self):
+        s = self.spec
+        Qp = np.empty(s.state_shape())
+        T = np.empty((s.nz, s.ny, s.nx))
+        mu = np.empty_like(T)
+        sen = np.empty_like(T)
+        lib().oxo_get_primitives(self.h, _dp(Qp), _dp(T), _dp(mu), _dp(sen))
+        return Qp, T, mu, sen
+
+    def diagnostics(self):
+        d = _abi.OxDiag()
+        lib().oxo_diagnostics(self.h, ctypes.byref(d))
+        return d.as_dict()

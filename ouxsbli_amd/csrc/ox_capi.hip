@@ -1,0 +1,426 @@
+// ox_capi.hip -- the C ABI of include/ouxsbli.h: handle lifecycle, state
+// transfer, the RK driver, diagnostics.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ox_core.hpp"
+#include "ox_diag.hpp"
+#include "ox_kernels.hpp"
+
+namespace ox {
+
+// ---------------------------------------------------------------------------
+// Periodic ghost fill: every ghost cell takes its periodic image (the net
+// effect of set_bc_cyclic{2,4,6}, 3D_solver/src/set_bc_common.f90:101-273).
+// Sources are interior cells only, so one pass has no ordering hazard.
+// Thread mapping: linear index over the ghost shell:
+//   [0, A)        z slabs  (2h planes of nx*ny)
+//   [A, A+B)      y slabs  (iz rows-planes of 2h*nx)
+//   [A+B, A+B+C)  x slabs  (iz*iy rows of 2h)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ghost_fill(double* __restrict__ Q, Geom g) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int h = g.h, nx = g.nx, ny = g.ny;
+  const long long A = 2LL * h * nx * ny;
+  const long long B = (long long)g.iz * 2 * h * nx;
+  const long long C = (long long)g.iz * g.iy * 2 * h;
+  int i, j, k;
+  if (t < A) {
+    const int pl = (int)(t / ((long long)nx * ny));
+    const int r = (int)(t % ((long long)nx * ny));
+    i = r % nx;
+    j = r / nx;
+    k = pl < h ? pl : g.nz - 2 * h + pl;
+  } else if (t < A + B) {
+    const long long u = t - A;
+    const int row = (int)(u / nx);
+    i = (int)(u % nx);
+    k = h + row / (2 * h);
+    const int s = row % (2 * h);
+    j = s < h ? s : g.ny - 2 * h + s;
+  } else if (t < A + B + C) {
+    const long long u = t - A - B;
+    const int s = (int)(u % (2 * h));
+    const long long row = u / (2 * h);
+    j = h + (int)(row % g.iy);
+    k = h + (int)(row / g.iy);
+    i = s < h ? s : nx - 2 * h + s;
+  } else {
+    return;
+  }
+  // periodic image inside the interior [h, h+n)
+  const int si = i < h ? i + g.ix : (i >= h + g.ix ? i - g.ix : i);
+  const int sj = j < h ? j + g.iy : (j >= h + g.iy ? j - g.iy : j);
+  const int sk = k < h ? k + g.iz : (k >= h + g.iz ? k - g.iz : k);
+#pragma unroll
+  for (int l = 0; l < 5; ++l) Q[qidx(g, i, l, j, k)] = Q[qidx(g, si, l, sj, sk)];
+}
+
+// QJ = Q / J (pre_calc, 3D_solver/src/preprocess.f90.fypp:65-71) and back.
+__global__ __launch_bounds__(256) void k_scale(double* __restrict__ Q, long long n, double J, int divide) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  Q[t] = divide ? Q[t] / J : J * Q[t];
+}
+
+}  // namespace ox
+
+struct ox_solver {
+  ox::Core C;
+  double* QJ = nullptr;   // state Q^n
+  double* QJ2 = nullptr;  // QJ2 (RK3) / QJs (RK4)
+  double* Rs = nullptr;   // RK4 accumulator
+  double* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  ox::StageFn stage = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define OX_HIP(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) return fail(OX_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+size_t state_elems(const ox::Geom& g) { return (size_t)g.nx * 5 * g.ny * g.nz; }
+size_t cell_elems(const ox::Geom& g) { return (size_t)g.nx * g.ny * g.nz; }
+
+int validate(const ox_config& c) {
+  if (c.abi_version != OX_ABI_VERSION) return fail(OX_EINVAL, "abi_version mismatch");
+  if (c.order != 2 && c.order != 4 && c.order != 6) return fail(OX_EINVAL, "order must be 2, 4 or 6");
+  if (c.rk != 3 && c.rk != 4) return fail(OX_EINVAL, "rk must be 3 or 4");
+  if (c.bc_x || c.bc_y || c.bc_z) return fail(OX_EUNSUPPORTED, "only periodic boundaries are built");
+  if (c.visc == OX_VISC_LES) return fail(OX_EUNSUPPORTED, "LES is not built yet");
+  if (c.visc_stencil != OX_STENCIL_CENTRAL && c.visc != OX_VISC_EULER)
+    return fail(OX_EUNSUPPORTED, "ME4Base viscous stencil is not built yet");
+  if (c.scheme != OX_SCHEME_KEEP && c.scheme != OX_SCHEME_SLAU)
+    return fail(OX_EUNSUPPORTED, "scheme not built (KEEP and SLAU are)");
+  if (c.tvd < 0 || c.tvd > 2) return fail(OX_EINVAL, "tvd must be 0, 1 or 2");
+  if (c.slau_variant < 0 || c.slau_variant > 1) return fail(OX_EINVAL, "slau_variant must be 0 or 1");
+  if (c.precision != OX_PREC_FP64 && c.precision != OX_PREC_FP32_BULK) return fail(OX_EINVAL, "bad precision");
+  const int h = c.order / 2;
+  if (c.nx < 2 * h + 2 * c.order || c.ny < 2 * h + 2 * c.order || c.nz < 2 * h + 2 * c.order)
+    return fail(OX_EINVAL, "grid too small for the stencil");
+  if ((double)c.nx * 5.0 * c.ny * c.nz >= 2147483647.0) return fail(OX_EINVAL, "grid too large for 32-bit indexing");
+  if (c.nz > 65535 || c.ny > 65535 * 4) return fail(OX_EINVAL, "grid extent exceeds launch limits");
+  if (!(c.dt > 0.0) || !(c.Lx > 0.0) || !(c.Ly > 0.0) || !(c.Lz > 0.0) || !(c.gamma > 1.0) || !(c.R > 0.0))
+    return fail(OX_EINVAL, "non-positive dt, box length, gamma-1 or R");
+  return OX_OK;
+}
+
+// set_grid_cyclic / set_Jacobian_xy3 / pre_calc metric scalars for a uniform
+// periodic grid (src/set_coordinate.f90:99-339, preprocess.f90.fypp:72-87).
+ox::Geom make_geom(const ox_config& c) {
+  ox::Geom g;
+  g.nx = c.nx; g.ny = c.ny; g.nz = c.nz;
+  g.h = c.order / 2;
+  g.ix = c.nx - 2 * g.h; g.iy = c.ny - 2 * g.h; g.iz = c.nz - 2 * g.h;
+  const double dx = c.Lx / (double)(c.nx - c.order);
+  const double dy = c.Ly / (double)(c.ny - c.order);
+  const double dz = c.Lz / (double)(c.nz - c.order);
+  g.jac = 8.0 / ((dx + dx) * (dy + dy) * 2.0 * dz);
+  g.inv_dx = 1.0 / dx;
+  g.inv_dy = 1.0 / dy;
+  g.inv_dz = 1.0 / dz;
+  g.a_xy = 0.25 * c.dt * (dx + dx) * (dy + dy);
+  g.a_yz = 0.25 * c.dt * (dy + dy) * (dz + dz);
+  g.a_zx = 0.25 * c.dt * (dz + dz) * (dx + dx);
+  return g;
+}
+
+void destroy(ox_solver* s) {
+  if (!s) return;
+  hipFree(s->QJ);
+  hipFree(s->QJ2);
+  hipFree(s->Rs);
+  hipFree(s->scratch);
+  hipFree(s->C.Racc);
+  for (int m = 0; m < 8; ++m) hipFree(s->C.prim[m]);
+  if (s->ev0) hipEventDestroy(s->ev0);
+  if (s->ev1) hipEventDestroy(s->ev1);
+  if (s->C.stream) hipStreamDestroy(s->C.stream);
+  delete s;
+}
+
+int run_rk_step(ox_solver* s) {
+  ox::Core& C = s->C;
+  const ox::Geom& g = C.g;
+  ox::StageArgs sa{};
+  if (C.cfg.rk == 3) {
+    // calc_time_dev.f90.fypp:86-100
+    sa = {s->QJ, s->QJ2, s->Rs, 0, 0, 0, ox::EP_RK3_S1};
+    s->stage(C, s->QJ, sa, 1.0 * g.a_yz, 1.0 * g.a_zx, 1.0 * g.a_xy);
+    sa = {s->QJ, s->QJ2, s->Rs, 0.75, 0.25, 1.0, ox::EP_RK3_S23};
+    s->stage(C, s->QJ2, sa, 0.25 * g.a_yz, 0.25 * g.a_zx, 0.25 * g.a_xy);
+    sa = {s->QJ2, s->QJ, s->Rs, 2.0, 1.0, C.hc.one_third, ox::EP_RK3_S23};
+    s->stage(C, s->QJ2, sa, 2.0 * g.a_yz, 2.0 * g.a_zx, 2.0 * g.a_xy);
+  } else {
+    // calc_time_dev.f90.fypp:268-285
+    sa = {s->QJ, s->QJ2, s->Rs, 0.5, 1.0, 0, ox::EP_RK4_S};
+    s->stage(C, s->QJ, sa, g.a_yz, g.a_zx, g.a_xy);
+    sa = {s->QJ, s->QJ2, s->Rs, 0.5, 2.0, 0, ox::EP_RK4_S};
+    s->stage(C, s->QJ2, sa, g.a_yz, g.a_zx, g.a_xy);
+    sa = {s->QJ, s->QJ2, s->Rs, 1.0, 2.0, 0, ox::EP_RK4_S};
+    s->stage(C, s->QJ2, sa, g.a_yz, g.a_zx, g.a_xy);
+    sa = {s->QJ, s->QJ, s->Rs, C.hc.one_sixth, 0, 0, ox::EP_RK4_S4};
+    s->stage(C, s->QJ2, sa, g.a_yz, g.a_zx, g.a_xy);
+  }
+  return OX_OK;
+}
+
+}  // namespace
+
+namespace ox {
+void launch_ghost_fill(Core& C, double* Q) {
+  const Geom& g = C.g;
+  const long long total = 2LL * g.h * g.nx * g.ny + (long long)g.iz * 2 * g.h * g.nx + (long long)g.iz * g.iy * 2 * g.h;
+  k_ghost_fill<<<(unsigned)((total + 255) / 256), 256, 0, C.stream>>>(Q, g);
+}
+}  // namespace ox
+
+extern "C" {
+
+const char* ox_last_error(void) { return g_err.c_str(); }
+
+const char* ox_version(void) { return "ouxsbli-mi355x 1 gfx950"; }
+
+int ox_create(const ox_config* cfg, ox_solver** out) {
+  if (!cfg || !out) return fail(OX_EINVAL, "null argument");
+  *out = nullptr;
+  int rc = validate(*cfg);
+  if (rc != OX_OK) return rc;
+  ox_solver* s = new (std::nothrow) ox_solver();
+  if (!s) return fail(OX_ENOMEM, "host allocation failed");
+  ox::Core& C = s->C;
+  C.cfg = *cfg;
+  C.g = make_geom(*cfg);
+  C.hc.init(*cfg);
+  C.vorder = cfg->visc == OX_VISC_EULER ? 0 : (cfg->visc_order ? cfg->visc_order : cfg->order);
+  C.tvd = cfg->tvd;
+  C.variant = cfg->slau_variant;
+  C.Racc = nullptr;
+  for (int m = 0; m < 8; ++m) C.prim[m] = nullptr;
+  C.stream = nullptr;
+  if (C.vorder != 0 && C.vorder != 4 && C.vorder != 6) {
+    delete s;
+    return fail(OX_EINVAL, "visc_order must be 4 or 6");
+  }
+  if (C.vorder == 6 && cfg->order != 6) {
+    delete s;
+    return fail(OX_EINVAL, "visc_order 6 needs order 6 (three ghost layers)");
+  }
+  s->stage = cfg->precision == OX_PREC_FP64 ? ox::find_stage_fp64(cfg->order, cfg->scheme, cfg->visc, C.vorder)
+                                            : ox::find_stage_fp32(cfg->order, cfg->scheme, cfg->visc, C.vorder);
+  if (!s->stage) {
+    delete s;
+    return fail(OX_EUNSUPPORTED, "no kernel specialisation for this (order, scheme, visc, visc_order)");
+  }
+  hipError_t e = hipSetDevice(cfg->device);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(OX_EDEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  }
+  const size_t ns = state_elems(C.g), nc = cell_elems(C.g);
+  const size_t pe = cfg->precision == OX_PREC_FP64 ? sizeof(double) : sizeof(float);
+  bool ok = hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && hipMalloc(&s->QJ, ns * sizeof(double)) == hipSuccess;
+  ok = ok && hipMalloc(&s->QJ2, ns * sizeof(double)) == hipSuccess;
+  ok = ok && hipMalloc(&C.Racc, ns * sizeof(double)) == hipSuccess;
+  if (cfg->rk == 4) ok = ok && hipMalloc(&s->Rs, ns * sizeof(double)) == hipSuccess;
+  for (int m = 0; m < 8 && ok; ++m) ok = hipMalloc(&C.prim[m], nc * pe) == hipSuccess;
+  s->scratch_bytes = 4096 * (2 * ox::DIAG_NQ + 1) * sizeof(double);
+  ok = ok && hipMalloc(&s->scratch, s->scratch_bytes) == hipSuccess;
+  ok = ok && hipEventCreate(&s->ev0) == hipSuccess && hipEventCreate(&s->ev1) == hipSuccess;
+  if (!ok) {
+    destroy(s);
+    return fail(OX_ENOMEM, "device allocation failed");
+  }
+  // Zero everything once: ghost layers of QJ2 and the accumulators start defined.
+  ok = hipMemsetAsync(s->QJ, 0, ns * sizeof(double), C.stream) == hipSuccess;
+  ok = ok && hipMemsetAsync(s->QJ2, 0, ns * sizeof(double), C.stream) == hipSuccess;
+  ok = ok && hipMemsetAsync(C.Racc, 0, ns * sizeof(double), C.stream) == hipSuccess;
+  if (s->Rs) ok = ok && hipMemsetAsync(s->Rs, 0, ns * sizeof(double), C.stream) == hipSuccess;
+  for (int m = 0; m < 8 && ok; ++m) ok = hipMemsetAsync(C.prim[m], 0, nc * pe, C.stream) == hipSuccess;
+  ok = ok && hipStreamSynchronize(C.stream) == hipSuccess;
+  if (!ok) {
+    destroy(s);
+    return fail(OX_EDEVICE, "device initialisation failed");
+  }
+  *out = s;
+  return OX_OK;
+}
+
+void ox_destroy(ox_solver* s) {
+  if (!s) return;
+  hipSetDevice(s->C.cfg.device);
+  if (s->C.stream) hipStreamSynchronize(s->C.stream);
+  destroy(s);
+}
+
+int ox_set_qj(ox_solver* s, const double* QJ) {
+  if (!s || !QJ) return fail(OX_EINVAL, "null argument");
+  OX_HIP(hipSetDevice(s->C.cfg.device));
+  const size_t n = state_elems(s->C.g);
+  OX_HIP(hipMemcpyAsync(s->QJ, QJ, n * sizeof(double), hipMemcpyHostToDevice, s->C.stream));
+  if (s->Rs) OX_HIP(hipMemsetAsync(s->Rs, 0, n * sizeof(double), s->C.stream));
+  OX_HIP(hipStreamSynchronize(s->C.stream));
+  return OX_OK;
+}
+
+int ox_get_qj(ox_solver* s, double* QJ) {
+  if (!s || !QJ) return fail(OX_EINVAL, "null argument");
+  OX_HIP(hipSetDevice(s->C.cfg.device));
+  const size_t n = state_elems(s->C.g);
+  OX_HIP(hipMemcpyAsync(QJ, s->QJ, n * sizeof(double), hipMemcpyDeviceToHost, s->C.stream));
+  OX_HIP(hipStreamSynchronize(s->C.stream));
+  return OX_OK;
+}
+
+int ox_set_state(ox_solver* s, const double* Q) {
+  if (!s || !Q) return fail(OX_EINVAL, "null argument");
+  int rc = ox_set_qj(s, Q);
+  if (rc != OX_OK) return rc;
+  const long long n = (long long)state_elems(s->C.g);
+  ox::k_scale<<<(unsigned)((n + 255) / 256), 256, 0, s->C.stream>>>(s->QJ, n, s->C.g.jac, 1);
+  OX_HIP(hipGetLastError());
+  OX_HIP(hipStreamSynchronize(s->C.stream));
+  return OX_OK;
+}
+
+int ox_get_state(ox_solver* s, double* Q) {
+  if (!s || !Q) return fail(OX_EINVAL, "null argument");
+  OX_HIP(hipSetDevice(s->C.cfg.device));
+  const long long n = (long long)state_elems(s->C.g);
+  // J * QJ into the scratch-free second state buffer is not allowed (RK state);
+  // scale on the host instead.
+  OX_HIP(hipMemcpyAsync(Q, s->QJ, n * sizeof(double), hipMemcpyDeviceToHost, s->C.stream));
+  OX_HIP(hipStreamSynchronize(s->C.stream));
+  const double J = s->C.g.jac;
+  for (long long t = 0; t < n; ++t) Q[t] = J * Q[t];
+  return OX_OK;
+}
+
+int ox_step(ox_solver* s, int32_t nsteps) {
+  if (!s) return fail(OX_EINVAL, "null handle");
+  if (nsteps < 0) return fail(OX_EINVAL, "nsteps < 0");
+  OX_HIP(hipSetDevice(s->C.cfg.device));
+  OX_HIP(hipEventRecord(s->ev0, s->C.stream));
+  for (int t = 0; t < nsteps; ++t) run_rk_step(s);
+  OX_HIP(hipGetLastError());
+  OX_HIP(hipEventRecord(s->ev1, s->C.stream));
+  OX_HIP(hipEventSynchronize(s->ev1));
+  float ms = 0.f;
+  OX_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  s->last_ms = ms;
+  return OX_OK;
+}
+
+int ox_rhs(ox_solver* s, double* R) {
+  if (!s || !R) return fail(OX_EINVAL, "null argument");
+  OX_HIP(hipSetDevice(s->C.cfg.device));
+  const ox::Geom& g = s->C.g;
+  const size_t n = state_elems(g);
+  // the result goes to the second state buffer, which is then restored from
+  // Q^n semantics: QJ2 is scratch between steps.
+  OX_HIP(hipMemsetAsync(s->QJ2, 0, n * sizeof(double), s->C.stream));
+  ox::StageArgs sa{s->QJ, s->QJ2, s->Rs, 0, 0, 0, ox::EP_RHS};
+  s->stage(s->C, s->QJ, sa, g.a_yz, g.a_zx, g.a_xy);
+  OX_HIP(hipGetLastError());
+  OX_HIP(hipMemcpyAsync(R, s->QJ2, n * sizeof(double), hipMemcpyDeviceToHost, s->C.stream));
+  OX_HIP(hipStreamSynchronize(s->C.stream));
+  return OX_OK;
+}
+
+int ox_diagnostics(ox_solver* s, ox_diag* out) {
+  if (!s || !out) return fail(OX_EINVAL, "null argument");
+  OX_HIP(hipSetDevice(s->C.cfg.device));
+  const ox::Geom& g = s->C.g;
+  const long long n = (long long)g.ix * g.iy * g.iz;
+  int nb = (int)std::min<long long>(4096, (n + ox::DIAG_BLOCK - 1) / ox::DIAG_BLOCK);
+  if (nb < 1) nb = 1;
+  ox::k_diag<<<nb, ox::DIAG_BLOCK, 0, s->C.stream>>>(s->QJ, g, s->C.cfg.gamma, s->scratch);
+  OX_HIP(hipGetLastError());
+  std::vector<double> part((size_t)nb * (2 * ox::DIAG_NQ + 1));
+  OX_HIP(hipMemcpyAsync(part.data(), s->scratch, part.size() * sizeof(double), hipMemcpyDeviceToHost, s->C.stream));
+  OX_HIP(hipStreamSynchronize(s->C.stream));
+  double sum[ox::DIAG_NQ], comp[ox::DIAG_NQ];
+  for (int q = 0; q < ox::DIAG_NQ; ++q) sum[q] = comp[q] = 0.0;
+  double wmax = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    const double* p = part.data() + (size_t)b * (2 * ox::DIAG_NQ + 1);
+    for (int q = 0; q < ox::DIAG_NQ; ++q) {
+      const double x = p[2 * q];
+      const double t = sum[q] + x;
+      if (std::fabs(sum[q]) >= std::fabs(x)) comp[q] += (sum[q] - t) + x;
+      else comp[q] += (x - t) + sum[q];
+      sum[q] = t;
+      comp[q] += p[2 * q + 1];
+    }
+    wmax = std::max(wmax, p[2 * ox::DIAG_NQ]);
+  }
+  const double ncell = (double)g.ix * (double)g.iy * (double)g.iz;
+  out->ke = (sum[0] + comp[0]) / ncell;
+  out->entropy = (sum[1] + comp[1]) / ncell;
+  out->enstrophy = (sum[2] + comp[2]) / ncell;
+  out->max_wave_speed = wmax;
+  out->mass = sum[3] + comp[3];
+  out->energy = sum[4] + comp[4];
+  return OX_OK;
+}
+
+int ox_check_state(ox_solver* s, int64_t* bad_index, int64_t* count) {
+  if (!s) return fail(OX_EINVAL, "null handle");
+  OX_HIP(hipSetDevice(s->C.cfg.device));
+  const ox::Geom& g = s->C.g;
+  const long long n = (long long)g.ix * g.iy * g.iz;
+  int nb = (int)std::min<long long>(4096, (n + ox::DIAG_BLOCK - 1) / ox::DIAG_BLOCK);
+  if (nb < 1) nb = 1;
+  long long* part = reinterpret_cast<long long*>(s->scratch);
+  ox::k_check<<<nb, ox::DIAG_BLOCK, 0, s->C.stream>>>(s->QJ, g, s->C.cfg.gamma, part);
+  OX_HIP(hipGetLastError());
+  std::vector<long long> h((size_t)2 * nb);
+  OX_HIP(hipMemcpyAsync(h.data(), part, h.size() * sizeof(long long), hipMemcpyDeviceToHost, s->C.stream));
+  OX_HIP(hipStreamSynchronize(s->C.stream));
+  long long cnt = 0, first = -1;
+  for (int b = 0; b < nb; ++b) {
+    cnt += h[2 * b];
+    const long long f = h[2 * b + 1];
+    if (f >= 0 && (first < 0 || f < first)) first = f;
+  }
+  if (bad_index) *bad_index = first;
+  if (count) *count = cnt;
+  if (cnt > 0) return fail(OX_ESTATE, "non-physical state: " + std::to_string(cnt) + " bad cells, first at linear index " +
+                                          std::to_string(first));
+  return OX_OK;
+}
+
+int ox_synchronize(ox_solver* s) {
+  if (!s) return fail(OX_EINVAL, "null handle");
+  OX_HIP(hipSetDevice(s->C.cfg.device));
+  OX_HIP(hipStreamSynchronize(s->C.stream));
+  return OX_OK;
+}
+
+int ox_last_step_ms(ox_solver* s, double* ms) {
+  if (!s || !ms) return fail(OX_EINVAL, "null argument");
+  *ms = s->last_ms;
+  return OX_OK;
+}
+
+}  // extern "C"

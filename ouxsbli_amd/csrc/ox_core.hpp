@@ -1,0 +1,34 @@
+// ox_core.hpp -- the per-handle device context shared by the C ABI
+// (ox_capi.hip) and the stage launchers (ox_fp64.hip, ox_fp32.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ox_physics.hpp"
+#include "ox_types.hpp"
+
+namespace ox {
+
+struct Core {
+  ox_config cfg;
+  Geom g;
+  HostConsts hc;
+  int vorder;        // viscous order actually used (0 for Euler)
+  int tvd, variant;  // runtime switches (uniform across the launch)
+  // primitive arrays (element type per precision mode) and the accumulator
+  void* prim[8];     // rho,u,v,w,p,T,mu,sensor
+  double* Racc;
+  hipStream_t stream;
+};
+
+// Launches one right-hand-side evaluation on state Qsrc followed by the
+// stage epilogue `sa` (ghost fill of sa.Qout included unless EP_RHS).
+// cx, cy, cz are the calc_R coefficients (coef * dt * face area).
+using StageFn = void (*)(Core& C, const double* Qsrc, const StageArgs& sa, double cx, double cy, double cz);
+
+StageFn find_stage_fp64(int order, int scheme, int visc, int vorder);
+StageFn find_stage_fp32(int order, int scheme, int visc, int vorder);
+
+void launch_ghost_fill(Core& C, double* Q);
+
+}  // namespace ox

@@ -1,0 +1,338 @@
+// ox_kernels.hpp -- gfx950 kernels of one RK stage (templated; instantiated
+// by ox_fp64.hip / ox_fp32.hip).
+//
+// Data layout in HBM (all on one GPU, ghosts included):
+//   state  QJ(nx,5,ny,nz) FP64, the reference layout (x fastest, then the
+//          variable, then y, z): drop-in for set_state/get_state/checkpoints.
+//   prims  rho,u,v,w,p,T,mu,sensor as separate (nx,ny,nz) arrays of type S
+//          (FP64 plain form, FP32 in the FP32-bulk form).
+//   Racc   residual accumulator in the state layout, FP64.
+//
+// One stage = k_prim (QJ -> primitives, whole grid) [+ k_sensor] ->
+// k_sweep_x (LDS row tiles; writes R = c_x*dE) -> k_sweep_line<DIR=1>
+// (register sliding window along y; R += c_y*dF) -> k_sweep_line<DIR=2>
+// (sliding window along z; R += c_z*dG, then the RK combination) ->
+// k_ghost_fill.  Face fluxes (convective minus viscous) live only in
+// registers: E, F and G are never written to memory.  The summation order
+// x, then y, then z is the order of calc_R (3D_solver/src/calc_steps.f90:8-38).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ox_physics.hpp"
+#include "ox_types.hpp"
+
+namespace ox {
+
+template <class S> struct Prim {
+  S *rho, *u, *v, *w, *p, *T, *mu, *s;
+};
+
+constexpr int KEEP = OX_SCHEME_KEEP;
+constexpr int SLAU = OX_SCHEME_SLAU;
+
+// Stencil width of a face: the convective stencil (ORDER cells) and the
+// viscous one (VO cells) are both centred on the face.
+template <int ORDER, int VISC, int VO> struct Sten {
+  static constexpr int SW = (VISC && VO > ORDER) ? VO : ORDER;
+  static constexpr int HW = SW / 2;
+  static constexpr int CO = HW - ORDER / 2;  // first convective cell in the window
+  static constexpr int VOFF = VISC ? HW - VO / 2 : 0;
+  static constexpr int VH = VO / 2;  // half width of the cross-derivative stencil
+};
+
+// state index of (i,l,j,k), 0-based
+__device__ __forceinline__ int qidx(const Geom& g, int i, int l, int j, int k) {
+  return i + g.nx * (l + 5 * (j + g.ny * k));
+}
+__device__ __forceinline__ int cidx(const Geom& g, int i, int j, int k) { return i + g.nx * (j + g.ny * k); }
+
+// ---------------------------------------------------------------------------
+// Primitive recovery.  calc_quantities_T_3D / calc_quantities_3D
+// (src/calc_physical_quantities.f90:60-116), always in FP64.
+// ---------------------------------------------------------------------------
+template <class S, bool VISC>
+__global__ __launch_bounds__(256) void k_prim(const double* __restrict__ QJ, Prim<S> P, Geom g, PhysC<double> c) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.nx || j >= g.ny) return;
+  const int q = qidx(g, i, 0, j, k);
+  const int st = g.nx;
+  const double J = g.jac;
+  const double q1 = QJ[q], q2 = QJ[q + st], q3 = QJ[q + 2 * st], q4 = QJ[q + 3 * st], q5 = QJ[q + 4 * st];
+  const double over_Q1 = 1.0 / q1;
+  const double rho = J * q1;
+  const double u = q2 * over_Q1;
+  const double v = q3 * over_Q1;
+  const double w = q4 * over_Q1;
+  const double p = c.gamma_1 * (J * q5 - 0.5 * rho * (u * u + v * v + w * w));
+  const double T = p / (c.Rgas * rho);
+  const int m = cidx(g, i, j, k);
+  P.rho[m] = (S)rho;
+  P.u[m] = (S)u;
+  P.v[m] = (S)v;
+  P.w[m] = (S)w;
+  P.p[m] = (S)p;
+  P.T[m] = (S)T;
+  if constexpr (VISC) P.mu[m] = (S)(c.mu0c / (T + 111.0) * (T * fsqrt(T)));
+}
+
+// ---------------------------------------------------------------------------
+// Ducros sensor.  calc_Ducros (3D_solver/src/calc_hybrid.f90:11-72), over
+// the cells the reference computes (Fortran 2..n-1); the boundary copies are
+// never read by the periodic kernels.
+// ---------------------------------------------------------------------------
+template <class S, class R>
+__global__ __launch_bounds__(256) void k_sensor(Prim<S> P, Geom g) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.nx - 2 || j > g.ny - 2) return;
+  const int m = cidx(g, i, j, k);
+  const int sy = g.nx, sz = g.nx * g.ny;
+  const R dxt = R(0.25) * (R(g.inv_dx) + R(g.inv_dx));
+  const R dyt = R(0.25) * (R(g.inv_dy) + R(g.inv_dy));
+  const R dzt = R(0.25) * (R(g.inv_dz) + R(g.inv_dz));
+  const R dudx = (-R(P.u[m - 1]) + R(P.u[m + 1])) * dxt;
+  const R dvdx = (-R(P.v[m - 1]) + R(P.v[m + 1])) * dxt;
+  const R dwdx = (-R(P.w[m - 1]) + R(P.w[m + 1])) * dxt;
+  const R dudy = (-R(P.u[m - sy]) + R(P.u[m + sy])) * dyt;
+  const R dvdy = (-R(P.v[m - sy]) + R(P.v[m + sy])) * dyt;
+  const R dwdy = (-R(P.w[m - sy]) + R(P.w[m + sy])) * dyt;
+  const R dudz = (-R(P.u[m - sz]) + R(P.u[m + sz])) * dzt;
+  const R dvdz = (-R(P.v[m - sz]) + R(P.v[m + sz])) * dzt;
+  const R dwdz = (-R(P.w[m - sz]) + R(P.w[m + sz])) * dzt;
+  const R div = dudx + dvdy + dwdz;
+  const R r1 = dwdy - dvdz;
+  const R r2 = dudz - dwdx;
+  const R r3 = dvdx - dudy;
+  R fd = (div * div) / (div * div + (r1 * r1 + r2 * r2 + r3 * r3) + R(1.0e-12));
+  fd = fmin2(R(1), fd);
+  P.s[m] = (S)fd;
+}
+
+// ---------------------------------------------------------------------------
+// One cell's worth of window data.
+// ---------------------------------------------------------------------------
+template <class R, int DIR, int ORDER, int SCHEME, int VISC, int VO> struct CellData {
+  R rho, u, v, w, p, T, mu, s, d1, d2, d3, d4;
+};
+
+// cross derivative of field f along axis `ax` at cell m (stride st)
+template <class R, class S, int VO>
+__device__ __forceinline__ R dcross(const PhysC<R>& c, const S* __restrict__ f, int m, int st, R inv) {
+  R a[VO + 1];
+#pragma unroll
+  for (int t = 0; t <= VO; ++t) a[t] = (t == VO / 2) ? R(0) : R(f[m + (t - VO / 2) * st]);
+  return cross_d<R, VO>(c, inv, a);
+}
+
+// Loads the primitive values (and, for viscous runs, the cross derivatives a
+// face in direction DIR needs) of cell (i,j,k) = linear m.
+template <class R, class S, int DIR, int ORDER, int SCHEME, int VISC, int VO>
+__device__ __forceinline__ void load_cell(const Prim<S>& P, const Geom& g, const PhysC<R>& c, int m,
+                                          CellData<R, DIR, ORDER, SCHEME, VISC, VO>& d) {
+  d.rho = R(P.rho[m]);
+  d.u = R(P.u[m]);
+  d.v = R(P.v[m]);
+  d.w = R(P.w[m]);
+  d.p = R(P.p[m]);
+  if constexpr (SCHEME == KEEP || VISC) d.T = R(P.T[m]);
+  if constexpr (SCHEME == SLAU && ORDER > 2) d.s = R(P.s[m]);
+  if constexpr (VISC) {
+    d.mu = R(P.mu[m]);
+    const int sx = 1, sy = g.nx, sz = g.nx * g.ny;
+    if constexpr (DIR == 0) {  // uy, vy, uz, wz
+      d.d1 = dcross<R, S, VO>(c, P.u, m, sy, R(g.inv_dy));
+      d.d2 = dcross<R, S, VO>(c, P.v, m, sy, R(g.inv_dy));
+      d.d3 = dcross<R, S, VO>(c, P.u, m, sz, R(g.inv_dz));
+      d.d4 = dcross<R, S, VO>(c, P.w, m, sz, R(g.inv_dz));
+    } else if constexpr (DIR == 1) {  // ux, vx, vz, wz
+      d.d1 = dcross<R, S, VO>(c, P.u, m, sx, R(g.inv_dx));
+      d.d2 = dcross<R, S, VO>(c, P.v, m, sx, R(g.inv_dx));
+      d.d3 = dcross<R, S, VO>(c, P.v, m, sz, R(g.inv_dz));
+      d.d4 = dcross<R, S, VO>(c, P.w, m, sz, R(g.inv_dz));
+    } else {  // ux, wx, vy, wy
+      d.d1 = dcross<R, S, VO>(c, P.u, m, sx, R(g.inv_dx));
+      d.d2 = dcross<R, S, VO>(c, P.w, m, sx, R(g.inv_dx));
+      d.d3 = dcross<R, S, VO>(c, P.v, m, sy, R(g.inv_dy));
+      d.d4 = dcross<R, S, VO>(c, P.w, m, sy, R(g.inv_dy));
+    }
+  }
+}
+
+// Face flux (convective minus viscous) from SW window cells w[0..SW-1]; the
+// face lies between w[HW-1] and w[HW].
+template <class R, int DIR, int ORDER, int SCHEME, int VISC, int VO>
+__device__ __forceinline__ void face_flux(const PhysC<R>& c, int tvd, int variant, R inv_d,
+                                          const CellData<R, DIR, ORDER, SCHEME, VISC, VO>* w, R* F) {
+  using St = Sten<ORDER, VISC, VO>;
+  constexpr int N = ORDER;
+  R rho[N], u[N], v[N], wv[N], p[N], T[N];
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    rho[m] = w[St::CO + m].rho;
+    u[m] = w[St::CO + m].u;
+    v[m] = w[St::CO + m].v;
+    wv[m] = w[St::CO + m].w;
+    p[m] = w[St::CO + m].p;
+    if constexpr (SCHEME == KEEP) T[m] = w[St::CO + m].T;
+  }
+  if constexpr (SCHEME == KEEP) {
+    if constexpr (N == 2) keep2<R, DIR>(c, rho, u, v, wv, p, T, F);
+    else if constexpr (N == 4) keep4<R, DIR>(c, rho, u, v, wv, p, T, F);
+    else keep6<R, DIR>(c, rho, u, v, wv, p, T, F);
+  } else {
+    R fs = R(0);
+    if constexpr (N > 2) fs = R(0.5) * (w[St::HW - 1].s + w[St::HW].s);
+    slau_face<R, DIR, N>(c, tvd, variant, rho, u, v, wv, p, fs, F);
+  }
+  if constexpr (VISC) {
+    constexpr int M = VO;
+    R mu_[M], T_[M], u_[M], v_[M], w_[M], a1[M], a2[M], a3[M], a4[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const auto& x = w[St::VOFF + m];
+      mu_[m] = x.mu; T_[m] = x.T; u_[m] = x.u; v_[m] = x.v; w_[m] = x.w;
+      a1[m] = x.d1; a2[m] = x.d2; a3[m] = x.d3; a4[m] = x.d4;
+    }
+    R tau[3], en;
+    visc_face<R, DIR, M>(c, inv_d, mu_, T_, u_, v_, w_, a1, a2, a3, a4, tau, en);
+    F[1] = F[1] - tau[0];
+    F[2] = F[2] - tau[1];
+    F[3] = F[3] - tau[2];
+    F[4] = F[4] - en;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x sweep: one wavefront per grid row; 64 faces -> 63 cells per tile; the
+// right-face flux of each cell comes from the neighbouring lane by a
+// cross-lane shuffle.  Writes Racc = c_x * (E_right - E_left) (the first term
+// of calc_R).
+// ---------------------------------------------------------------------------
+constexpr int XT = 63;  // cells per x tile
+constexpr int XROWS = 4;
+
+template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
+__global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__ Racc, Geom g, PhysC<R> c, int tvd,
+                                                 int variant, double cx) {
+  using St = Sten<ORDER, VISC, VO>;
+  using CD = CellData<R, 0, ORDER, SCHEME, VISC, VO>;
+  constexpr int W = XT + St::SW;
+  __shared__ CD tile[XROWS][W];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = g.h + blockIdx.x * XT;
+  const int j = g.h + blockIdx.y * XROWS + ty;
+  const int k = g.h + blockIdx.z;
+  const bool rowok = j < g.h + g.iy;
+  for (int m = tx; m < W; m += 64) {
+    const int i = i0 - St::HW + m;
+    if (rowok && i < g.nx) load_cell<R, S, 0, ORDER, SCHEME, VISC, VO>(P, g, c, cidx(g, i, j, k), tile[ty][m]);
+  }
+  __syncthreads();
+  if (!rowok) return;
+  // face f = i0 - 1 + tx: window cells f-HW+1 .. f+HW = tile positions tx .. tx+SW-1
+  R F[5] = {R(0), R(0), R(0), R(0), R(0)};
+  const int f = i0 - 1 + tx;
+  const bool face_ok = f <= g.h + g.ix - 1;
+  CD win[St::SW];
+  if (face_ok) {
+#pragma unroll
+    for (int m = 0; m < St::SW; ++m) win[m] = tile[ty][tx + m];
+    face_flux<R, 0, ORDER, SCHEME, VISC, VO>(c, tvd, variant, R(g.inv_dx), win, F);
+  }
+  double Fr[5];
+#pragma unroll
+  for (int l = 0; l < 5; ++l) Fr[l] = __shfl_down((double)F[l], 1, 64);
+  const int i = i0 + tx;
+  if (tx < XT && i < g.h + g.ix) {
+#pragma unroll
+    for (int l = 0; l < 5; ++l) Racc[qidx(g, i, l, j, k)] = cx * (-(double)F[l] + Fr[l]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y / z sweeps: each thread owns one (x, transverse) column and marches a
+// segment of LSEG cells along DIR with a register window of SW cells; loads
+// are coalesced along x across the wavefront.  DIR=1 accumulates
+// R += c_y*(F_right - F_left); DIR=2 adds c_z*(G_right - G_left) and applies
+// the stage epilogue.
+// ---------------------------------------------------------------------------
+constexpr int LSEG = 32;
+
+template <class R, class S, int ORDER, int SCHEME, int VISC, int VO, int DIR>
+__global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, double* __restrict__ Racc, Geom g, PhysC<R> c,
+                                                    int tvd, int variant, double cd, StageArgs sa) {
+  using St = Sten<ORDER, VISC, VO>;
+  using CD = CellData<R, DIR, ORDER, SCHEME, VISC, VO>;
+  const int i = g.h + blockIdx.x * 64 + threadIdx.x;
+  const int t = g.h + blockIdx.y * 4 + threadIdx.y;  // k for DIR 1, j for DIR 2
+  const int s0 = g.h + blockIdx.z * LSEG;
+  const int nt = DIR == 1 ? g.iz : g.iy;
+  const int nd = DIR == 1 ? g.iy : g.iz;
+  if (i >= g.h + g.ix || t >= g.h + nt) return;
+  const int s_end = min(s0 + LSEG, g.h + nd);
+  const int stride = DIR == 1 ? g.nx : g.nx * g.ny;
+  const int mbase = DIR == 1 ? cidx(g, i, 0, t) : cidx(g, i, t, 0);
+  const int qbase = DIR == 1 ? qidx(g, i, 0, 0, t) : qidx(g, i, 0, t, 0);
+  const int qstride = DIR == 1 ? 5 * g.nx : 5 * g.nx * g.ny;
+  const R inv_d = DIR == 1 ? R(g.inv_dy) : R(g.inv_dz);
+  CD win[St::SW];
+#pragma unroll
+  for (int m = 0; m < St::SW - 1; ++m)
+    load_cell<R, S, DIR, ORDER, SCHEME, VISC, VO>(P, g, c, mbase + (s0 - St::HW + m) * stride, win[m]);
+  R Fp[5];
+  for (int f = s0 - 1; f < s_end; ++f) {
+    load_cell<R, S, DIR, ORDER, SCHEME, VISC, VO>(P, g, c, mbase + (f + St::HW) * stride, win[St::SW - 1]);
+    R F[5];
+    face_flux<R, DIR, ORDER, SCHEME, VISC, VO>(c, tvd, variant, inv_d, win, F);
+    if (f >= s0) {
+      const int q = qbase + f * qstride;  // cell f
+      double Rl[5];
+#pragma unroll
+      for (int l = 0; l < 5; ++l) Rl[l] = Racc[q + l * g.nx] + cd * (-(double)Fp[l] + (double)F[l]);
+      if constexpr (DIR == 1) {
+#pragma unroll
+        for (int l = 0; l < 5; ++l) Racc[q + l * g.nx] = Rl[l];
+      } else {
+        // stage epilogue (3D_solver/src/calc_steps.f90:60-69, :98-106, :137-145, :173-183)
+        if (sa.ep == EP_RK3_S1) {
+#pragma unroll
+          for (int l = 0; l < 5; ++l) sa.Qout[q + l * g.nx] = sa.Qin[q + l * g.nx] - Rl[l];
+        } else if (sa.ep == EP_RK3_S23) {
+#pragma unroll
+          for (int l = 0; l < 5; ++l) {
+            const int a = q + l * g.nx;
+            sa.Qout[a] = (sa.c1 * sa.Qin[a] + sa.c2 * sa.Qout[a] - Rl[l]) * sa.c4inv;
+          }
+        } else if (sa.ep == EP_RK4_S) {
+#pragma unroll
+          for (int l = 0; l < 5; ++l) {
+            const int a = q + l * g.nx;
+            sa.Qout[a] = sa.Qin[a] - sa.c1 * Rl[l];
+            sa.Rs[a] = sa.Rs[a] + sa.c2 * Rl[l];
+          }
+        } else if (sa.ep == EP_RK4_S4) {
+#pragma unroll
+          for (int l = 0; l < 5; ++l) {
+            const int a = q + l * g.nx;
+            const double r = sa.Rs[a] + Rl[l];
+            sa.Qout[a] = sa.Qout[a] - r * sa.c1;  // c1 = one_sixth
+            sa.Rs[a] = 0.0;
+          }
+        } else {
+#pragma unroll
+          for (int l = 0; l < 5; ++l) sa.Qout[q + l * g.nx] = Rl[l];
+        }
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < 5; ++l) Fp[l] = F[l];
+#pragma unroll
+    for (int m = 0; m < St::SW - 1; ++m) win[m] = win[m + 1];
+  }
+}
+
+}  // namespace ox

@@ -1,0 +1,353 @@
+// ox_physics.hpp -- device-side face physics for the gfx950 kernels.
+//
+// Convective split-form / upwind face fluxes, MUSCL reconstruction and the
+// central viscous stress at one face, written for stencils held in registers
+// (template-sized arrays, fully unrolled).  Each function cites the reference
+// routine whose arithmetic it reproduces; expression trees follow the
+// reference's left-to-right evaluation so that, built without FMA contraction,
+// the FP64 instantiation is bit-identical to a sequential evaluation of the
+// reference formulas.  `R` is the arithmetic type (double for the plain form,
+// float for the FP32-bulk form).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ox_types.hpp"
+
+namespace ox {
+
+// Fortran MIN/MAX as a select: the first operand wins ties, so the sign of a
+// zero result is well defined.
+template <class R> __device__ __forceinline__ R fmin2(R a, R b) { return (b < a) ? b : a; }
+template <class R> __device__ __forceinline__ R fmax2(R a, R b) { return (b > a) ? b : a; }
+// SIGN(1, x) with IEEE signed zero.
+template <class R> __device__ __forceinline__ R fsign1(R x) { return __builtin_copysign(R(1), x); }
+template <class R> __device__ __forceinline__ R fabs_(R x) { return __builtin_fabs(x); }
+__device__ __forceinline__ double fsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_sqrtf(x); }
+
+// Per-case constants in the arithmetic type R (src/mod_constant.f90.fypp:18-35).
+template <class R> struct PhysC {
+  R gamma, gamma_1, R_over_gamma_1, Rgas, Cp_over_Pr, mu0c;
+  R one_third, two_third, one_sixth, one_twelfth, one_sixty;
+  R one_24, one_48, one_60, one_120, one_240, seven_twelfth;
+  R gamma_over_gamma_1, minmod_b, threshold;
+};
+
+template <class R> __host__ PhysC<R> make_physc(const HostConsts& h) {
+  PhysC<R> c;
+  c.gamma = (R)h.gamma; c.gamma_1 = (R)h.gamma_1; c.R_over_gamma_1 = (R)h.R_over_gamma_1; c.Rgas = (R)h.Rgas;
+  c.Cp_over_Pr = (R)h.Cp_over_Pr; c.mu0c = (R)h.mu0c;
+  c.one_third = (R)h.one_third; c.two_third = (R)h.two_third; c.one_sixth = (R)h.one_sixth;
+  c.one_twelfth = (R)h.one_twelfth; c.one_sixty = (R)h.one_sixty;
+  c.one_24 = (R)h.one_24; c.one_48 = (R)h.one_48; c.one_60 = (R)h.one_60; c.one_120 = (R)h.one_120;
+  c.one_240 = (R)h.one_240; c.seven_twelfth = (R)h.seven_twelfth;
+  c.gamma_over_gamma_1 = (R)h.gamma_over_gamma_1; c.minmod_b = (R)h.minmod_b; c.threshold = (R)h.threshold;
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// KEEP split form.  3D_solver/src/calc_keep_3d.f90:1-111.  DIR selects the
+// face normal: uu is the normal velocity and the pressure term lands in
+// component 1+DIR (the literal `pres * Normal(m)` products are kept so that
+// signed zeros match the reference expression).
+// ---------------------------------------------------------------------------
+template <class R, int DIR>
+__device__ __forceinline__ void keep2(const PhysC<R>& c, const R* rho, const R* u, const R* v, const R* w,
+                                      const R* p, const R* T, R* F) {
+  const R* uu = DIR == 0 ? u : (DIR == 1 ? v : w);
+  const R N1 = DIR == 0 ? R(1) : R(0), N2 = DIR == 1 ? R(1) : R(0), N3 = DIR == 2 ? R(1) : R(0);
+  F[0] = R(0.25) * (rho[0] + rho[1]) * (uu[0] + uu[1]);
+  F[1] = R(0.5) * (F[0] * (u[0] + u[1]) + (p[0] + p[1]) * N1);
+  F[2] = R(0.5) * (F[0] * (v[0] + v[1]) + (p[0] + p[1]) * N2);
+  F[3] = R(0.5) * (F[0] * (w[0] + w[1]) + (p[0] + p[1]) * N3);
+  F[4] = F[0] * R(0.5) * (T[0] + T[1]) * c.R_over_gamma_1;
+  F[4] = F[4] + R(0.5) * (uu[0] * p[1] + uu[1] * p[0]);
+  F[4] = F[4] + R(0.5) * F[0] * (u[0] * u[1] + v[0] * v[1] + w[0] * w[1]);
+}
+
+template <class R, int DIR>
+__device__ __forceinline__ void keep4(const PhysC<R>& c, const R* rho, const R* u, const R* v, const R* w,
+                                      const R* p, const R* T, R* F) {
+  const R* uu = DIR == 0 ? u : (DIR == 1 ? v : w);
+  const R N1 = DIR == 0 ? R(1) : R(0), N2 = DIR == 1 ? R(1) : R(0), N3 = DIR == 2 ? R(1) : R(0);
+  R RV1 = (rho[1] + rho[2]) * (uu[1] + uu[2]);
+  R RV2 = (rho[1] + rho[3]) * (uu[1] + uu[3]);
+  R RV3 = (rho[0] + rho[2]) * (uu[0] + uu[2]);
+  F[0] = c.one_third * RV1 - (RV2 + RV3) * c.one_24;
+  RV1 = c.one_sixth * RV1;
+  RV2 = c.one_48 * RV2;
+  RV3 = c.one_48 * RV3;
+  const R A = RV1 - RV2;  // RV1_RV2
+  const R B = RV1 - RV3;  // RV1_RV3
+  const R pres = -(c.one_twelfth * p[0]) + c.seven_twelfth * (p[1] + p[2]) - c.one_twelfth * p[3];
+  F[1] = -(RV3 * u[0]) + A * u[1] + B * u[2] - RV2 * u[3] + pres * N1;
+  F[2] = -(RV3 * v[0]) + A * v[1] + B * v[2] - RV2 * v[3] + pres * N2;
+  F[3] = -(RV3 * w[0]) + A * w[1] + B * w[2] - RV2 * w[3] + pres * N3;
+  R ene = (-(RV3 * T[0]) + A * T[1] + B * T[2] - RV2 * T[3]) * c.R_over_gamma_1;
+  ene = ene + ((RV1 * (u[1] * u[2] + v[1] * v[2] + w[1] * w[2])) -
+               (RV2 * (u[1] * u[3] + v[1] * v[3] + w[1] * w[3]) + RV3 * (u[0] * u[2] + v[0] * v[2] + w[0] * w[2])));
+  F[4] = ene + (c.two_third * (uu[1] * p[2] + uu[2] * p[1]) -
+                c.one_twelfth * (uu[1] * p[3] + uu[3] * p[1] + uu[0] * p[2] + uu[2] * p[0]));
+}
+
+template <class R, int DIR>
+__device__ __forceinline__ void keep6(const PhysC<R>& c, const R* rho, const R* u, const R* v, const R* w,
+                                      const R* p, const R* T, R* F) {
+  const R* uu = DIR == 0 ? u : (DIR == 1 ? v : w);
+  const R N1 = DIR == 0 ? R(1) : R(0), N2 = DIR == 1 ? R(1) : R(0), N3 = DIR == 2 ? R(1) : R(0);
+  R RV1 = (rho[2] + rho[3]) * (uu[2] + uu[3]);
+  R RV2 = (rho[2] + rho[4]) * (uu[2] + uu[4]);
+  R RV3 = (rho[1] + rho[3]) * (uu[1] + uu[3]);
+  R RV4 = (rho[2] + rho[5]) * (uu[2] + uu[5]);
+  R RV5 = (rho[1] + rho[4]) * (uu[1] + uu[4]);
+  R RV6 = (rho[0] + rho[3]) * (uu[0] + uu[3]);
+  F[0] = R(0.375) * RV1 - R(0.075) * (RV2 + RV3) + (RV4 + RV5 + RV6) * c.one_120;
+  RV1 = R(0.1875) * RV1;
+  RV2 = R(0.0375) * RV2;
+  RV3 = R(0.0375) * RV3;
+  RV4 = RV4 * c.one_240;
+  RV5 = RV5 * c.one_240;
+  RV6 = RV6 * c.one_240;
+  const R a35 = -RV3 + RV5;
+  const R a124 = RV1 - RV2 + RV4;
+  const R a136 = RV1 - RV3 + RV6;
+  const R a25 = -RV2 + RV5;
+  const R pres = (p[0] - R(8) * p[1] + R(37) * (p[2] + p[3]) - R(8) * p[4] + p[5]) * c.one_60;
+  R ruu = RV6 * u[0] + a35 * u[1] + a124 * u[2] + a136 * u[3] + a25 * u[4] + RV4 * u[5];
+  F[1] = ruu + pres * N1;
+  R ruv = RV6 * v[0] + a35 * v[1] + a124 * v[2] + a136 * v[3] + a25 * v[4] + RV4 * v[5];
+  F[2] = ruv + pres * N2;
+  R ruw = RV6 * w[0] + a35 * w[1] + a124 * w[2] + a136 * w[3] + a25 * w[4] + RV4 * w[5];
+  F[3] = ruw + pres * N3;
+  R ene = (RV6 * T[0] + a35 * T[1] + a124 * T[2] + a136 * T[3] + a25 * T[4] + RV4 * T[5]) * c.R_over_gamma_1;
+  ene = ene + RV1 * (u[2] * u[3] + v[2] * v[3] + w[2] * w[3]) -
+        (RV2 * (u[2] * u[4] + v[2] * v[4] + w[2] * w[4]) + RV3 * (u[1] * u[3] + v[1] * v[3] + w[1] * w[3])) +
+        (RV4 * (u[2] * u[5] + v[2] * v[5] + w[2] * w[5]) + RV5 * (u[1] * u[4] + v[1] * v[4] + w[1] * w[4]) +
+         RV6 * (u[0] * u[3] + v[0] * v[3] + w[0] * w[3]));
+  F[4] = ene + (R(0.75) * (uu[2] * p[3] + uu[3] * p[2]) -
+                R(0.15) * (uu[2] * p[4] + uu[4] * p[2] + uu[1] * p[3] + uu[3] * p[1]) +
+                (uu[2] * p[5] + uu[5] * p[2] + uu[1] * p[4] + uu[4] * p[1] + uu[0] * p[3] + uu[3] * p[0]) * c.one_60);
+}
+
+// ---------------------------------------------------------------------------
+// MUSCL.  src/calc_muscl.f90:19-169.  tvd: OX_TVD_NONE / TVD / HYBRID.
+// ---------------------------------------------------------------------------
+template <class R> __device__ __forceinline__ R minmod2(R x, R y) {
+  const R sgn = fsign1(x);
+  return sgn * fmax2(fmin2(fabs_(x), sgn * y), R(0));
+}
+template <class R> __device__ __forceinline__ R minmod3(R x, R y, R z) {
+  const R sgn = fsign1(x);
+  return sgn * fmax2(fmin2(fmin2(fabs_(x), sgn * y), sgn * z), R(0));
+}
+template <class R> __device__ __forceinline__ R d33(R d1, R d2, R d3) {
+  return minmod3(d1, R(2) * d2, R(2) * d3) - R(2) * minmod3(d2, R(2) * d1, R(2) * d3) +
+         minmod3(d3, R(2) * d1, R(2) * d2);
+}
+
+// delta4 -> MUSCL3rd{nonTVD,Minmod,Threshold} (:44-91, :142-153)
+template <class R>
+__device__ __forceinline__ void delta4(const PhysC<R>& c, int tvd, R sensor, const R* a, R& al, R& ar) {
+  const R d1 = -a[0] + a[1];
+  const R d2 = -a[1] + a[2];
+  const R d3 = -a[2] + a[3];
+  if (tvd == 0 || (tvd == 2 && sensor < c.threshold)) {
+    al = a[1] + R(0.5) * (d1 + R(2) * d2) * c.one_third;
+    ar = a[2] - R(0.5) * (d3 + R(2) * d2) * c.one_third;
+  } else {
+    const R b = c.minmod_b;
+    const R dt1 = minmod2(d1, b * d2);
+    const R dt2 = minmod2(d2, b * d1);
+    al = a[1] + R(0.5) * (dt1 + R(2) * dt2) * c.one_third;
+    const R dt3 = minmod2(d3, b * d2);
+    const R dt4 = minmod2(d2, b * d3);
+    ar = a[2] - R(0.5) * (dt3 + R(2) * dt4) * c.one_third;
+  }
+}
+
+// delta6 -> MUSCL4th{nonTVD,TVD,Threshold} (:94-139, :156-169)
+template <class R>
+__device__ __forceinline__ void delta6(const PhysC<R>& c, int tvd, R sensor, const R* a, R& al, R& ar) {
+  const R d1 = -a[0] + a[1];
+  const R d2 = -a[1] + a[2];
+  const R d3 = -a[2] + a[3];
+  const R d4 = -a[3] + a[4];
+  const R d5 = -a[4] + a[5];
+  if (tvd == 0 || (tvd == 2 && sensor < c.threshold)) {
+    al = a[2] + (R(-0.4) * d1 + R(2.2) * d2 + R(4.8) * d3 - R(0.6) * d4) * c.one_twelfth;
+    ar = a[3] - (R(-0.6) * d2 + R(4.8) * d3 + R(2.2) * d4 - R(0.4) * d5) * c.one_twelfth;
+  } else {
+    const R delta2 = d3 - d33(d2, d3, d4) * c.one_sixth;
+    {
+      const R delta1 = d2 - d33(d1, d2, d3) * c.one_sixth;
+      const R dl = minmod2(delta1, R(4) * delta2);
+      const R dr = minmod2(delta2, R(4) * delta1);
+      al = a[2] + (dl + R(2) * dr) * c.one_sixth;
+    }
+    {
+      const R delta3 = d4 - d33(d3, d4, d5) * c.one_sixth;
+      const R dl = minmod2(delta2, R(4) * delta3);
+      const R dr = minmod2(delta3, R(4) * delta2);
+      ar = a[3] - (dr + R(2) * dl) * c.one_sixth;
+    }
+  }
+}
+
+// 3D_solver/src/calc_hybrid.f90:75-81
+template <class R> __device__ __forceinline__ R wiggle(const R* phi) {
+  const R phi1 = (-phi[0] + phi[1]) * (-phi[1] + phi[2]);
+  const R phi2 = (-phi[2] + phi[3]) * (-phi[1] + phi[2]);
+  return R(0.5) * (R(1) - fsign1(fmin2(phi1, phi2)));
+}
+
+// ---------------------------------------------------------------------------
+// SLAU1 / HRSLAU2.  3D_solver/src/calc_slau_3d.f90:1-104.
+// ---------------------------------------------------------------------------
+template <class R, int DIR>
+__device__ __forceinline__ void slau(const PhysC<R>& c, int variant, R rho1, R rho2, R u1, R u2, R v1, R v2, R w1,
+                                     R w2, R p1, R p2, R HR, R* F) {
+  const R un1 = DIR == 0 ? u1 : (DIR == 1 ? v1 : w1);
+  const R un2 = DIR == 0 ? u2 : (DIR == 1 ? v2 : w2);
+  const R N1 = DIR == 0 ? R(1) : R(0), N2 = DIR == 1 ? R(1) : R(0), N3 = DIR == 2 ? R(1) : R(0);
+  const R over_rho1 = R(1) / rho1;
+  const R over_rho2 = R(1) / rho2;
+  // SLAU_common
+  const R cl = fsqrt(c.gamma * p1 * over_rho1);
+  const R cr = fsqrt(c.gamma * p2 * over_rho2);
+  const R cc = R(0.5) * (cl + cr);
+  const R over_c = R(1) / cc;
+  const R Mp = un1 * over_c;
+  const R Mm = un2 * over_c;
+  const R g = -(fmax2(fmin2(Mp, R(0)), R(-1)) * fmin2(fmax2(Mm, R(0)), R(1)));
+  const R one_g_Vt = (R(1) - g) * (rho1 * fabs_(un1) + rho2 * fabs_(un2)) / (rho1 + rho2);
+  const R Vtp = one_g_Vt + g * fabs_(un1);
+  const R Vtm = one_g_Vt + g * fabs_(un2);
+  const R bp = (fabs_(Mp) < R(1)) ? R(0.25) * (R(2) - Mp) * ((Mp + R(1)) * (Mp + R(1))) : R(0.5) * (R(1) + fsign1(Mp));
+  const R bm = (fabs_(Mm) < R(1)) ? R(0.25) * (R(2) + Mm) * ((Mm - R(1)) * (Mm - R(1))) : R(0.5) * (R(1) + fsign1(-Mm));
+  const R dp = -p1 + p2;
+  const R k1 = R(0.5) * (u1 * u1 + v1 * v1 + w1 * w1);
+  const R k2 = R(0.5) * (u2 * u2 + v2 * v2 + w2 * w2);
+  R mass1, mass2, pres;
+  if (variant == 0) {  // SLAU1
+    const R M = fmin2(R(1), fsqrt(k1 + k2) * over_c);
+    const R x = (R(1) - M) * (R(1) - M);
+    const R mass = R(0.25) * (rho1 * (un1 + Vtp) + rho2 * (un2 - Vtm) - x * dp * over_c);
+    mass1 = mass + fabs_(mass);
+    mass2 = mass - fabs_(mass);
+    pres = R(0.5) * (p1 + p2 + (bp - bm) * (-dp) + (R(1) - x) * (bp + bm - R(1)) * (p1 + p2));
+  } else {  // HRSLAU2
+    const R Vec2 = fsqrt(k1 + k2);
+    const R M = fmin2(R(1), Vec2 * over_c);
+    const R x = (R(1) - M) * (R(1) - M);
+    const R mass = R(0.25) * (rho1 * (un1 + Vtp) + rho2 * (un2 - Vtm) - x * dp * over_c);
+    mass1 = mass + fabs_(mass);
+    mass2 = mass - fabs_(mass);
+    pres = R(0.5) * (p1 + p2 + (bp - bm) * (-dp) + HR * Vec2 * (bp + bm - R(1)) * R(0.5) * (rho1 + rho2) * cc);
+  }
+  F[0] = mass1 + mass2;
+  F[1] = mass1 * u1 + mass2 * u2 + pres * N1;
+  F[2] = mass1 * v1 + mass2 * v2 + pres * N2;
+  F[3] = mass1 * w1 + mass2 * w2 + pres * N3;
+  const R phi1 = (p1 * c.gamma_over_gamma_1 + rho1 * k1) * over_rho1;
+  const R phi2 = (p2 * c.gamma_over_gamma_1 + rho2 * k2) * over_rho2;
+  F[4] = mass1 * phi1 + mass2 * phi2;
+}
+
+// SLAU face: interp2/4/6 (calc_slau_kernel_internal.f90.fypp:31-81) then SLAU.
+// N = stencil length (2, 4, 6); fsens = face Ducros sensor.
+template <class R, int DIR, int N>
+__device__ __forceinline__ void slau_face(const PhysC<R>& c, int tvd, int variant, const R* rho, const R* u,
+                                          const R* v, const R* w, const R* p, R fsens, R* F) {
+  R rl, rr, ul, ur, vl, vr, wl, wr, pl, pr, hr;
+  if constexpr (N == 2) {
+    rl = rho[0]; rr = rho[1]; ul = u[0]; ur = u[1]; vl = v[0]; vr = v[1]; wl = w[0]; wr = w[1]; pl = p[0]; pr = p[1];
+    hr = R(1);
+  } else if constexpr (N == 4) {
+    delta4(c, tvd, fsens, rho, rl, rr);
+    delta4(c, tvd, fsens, u, ul, ur);
+    delta4(c, tvd, fsens, v, vl, vr);
+    delta4(c, tvd, fsens, w, wl, wr);
+    delta4(c, tvd, fsens, p, pl, pr);
+    hr = wiggle(p);
+  } else {
+    delta6(c, tvd, fsens, rho, rl, rr);
+    delta6(c, tvd, fsens, u, ul, ur);
+    delta6(c, tvd, fsens, v, vl, vr);
+    delta6(c, tvd, fsens, w, wl, wr);
+    delta6(c, tvd, fsens, p, pl, pr);
+    hr = wiggle(p + 1);
+  }
+  slau<R, DIR>(c, variant, rl, rr, ul, ur, vl, vr, wl, wr, pl, pr, hr, F);
+}
+
+// ---------------------------------------------------------------------------
+// Central viscous face.  3D_solver/src/calc_visc_cent.f90.fypp:2-91 with the
+// face assembly of calc_visc_high.f90.fypp:114-140 (VO=4) and
+// calc_visc_high_internal.f90.fypp:185-211 (VO=6) and their F/G counterparts.
+// d1..d4 are the cross derivatives the reference stages in LDS:
+//   DIR 0 (E): uy, vy, uz, wz     DIR 1 (F): ux, vx, vz, wz
+//   DIR 2 (G): ux, wx, vy, wy
+// Returns tau[3] (subtracted from components 2..4) and the energy term.
+// ---------------------------------------------------------------------------
+template <class R, int N> __device__ __forceinline__ R vinterp(const R* a) {
+  if constexpr (N == 4) return R(0.0625) * (-a[0] + R(9) * (a[1] + a[2]) - a[3]);
+  else return R(0.00390625) * (R(3) * a[0] - R(25) * a[1] + R(150) * (a[2] + a[3]) - R(25) * a[4] + R(3) * a[5]);
+}
+template <class R, int N> __device__ __forceinline__ R vdiff(const PhysC<R>& c, R d, const R* a) {
+  if constexpr (N == 4) return (a[0] - R(27) * (a[1] - a[2]) - a[3]) * c.one_24 * d;
+  else
+    return (R(-0.0046875) * a[0] + R(0.065104167) * a[1] - R(1.171875) * (a[2] - a[3]) - R(0.065104167) * a[4] +
+            R(0.0046875) * a[5]) *
+           d;
+}
+
+template <class R, int DIR, int N>
+__device__ __forceinline__ void visc_face(const PhysC<R>& c, R d, const R* mu, const R* T, const R* u, const R* v,
+                                          const R* w, const R* d1, const R* d2, const R* d3, const R* d4, R* tau,
+                                          R& energy) {
+  const R mui = vinterp<R, N>(mu);
+  const R kT = c.Cp_over_Pr * mui * vdiff<R, N>(c, d, T);
+  // straight component: (vel_DIR, a, b); cross components: (vel_m, g_m)
+  const R* vs = DIR == 0 ? u : (DIR == 1 ? v : w);
+  const R* sa = DIR == 0 ? d2 : (DIR == 1 ? d4 : d1);  // vy | wz | ux
+  const R* sb = DIR == 0 ? d4 : (DIR == 1 ? d1 : d3);  // wz | ux | vy
+  const R ai = vinterp<R, N>(sa);
+  const R bi = vinterp<R, N>(sb);
+  const R dn = vdiff<R, N>(c, d, vs);
+  const R ts = c.two_third * mui * (R(2) * dn - ai - bi);
+  const R uts = vinterp<R, N>(vs) * ts;
+  // the two cross components in component order
+  const R* va = DIR == 0 ? v : u;                      // first cross velocity
+  const R* ga = DIR == 0 ? d1 : d2;                    // uy | vx | wx
+  const R* vb = DIR == 2 ? v : w;                      // second cross velocity
+  const R* gb = DIR == 0 ? d3 : (DIR == 1 ? d3 : d4);  // uz | vz | wy
+  const R ta = mui * (vinterp<R, N>(ga) + vdiff<R, N>(c, d, va));
+  const R uta = vinterp<R, N>(va) * ta;
+  const R tb = mui * (vinterp<R, N>(gb) + vdiff<R, N>(c, d, vb));
+  const R utb = vinterp<R, N>(vb) * tb;
+  if constexpr (DIR == 0) {
+    tau[0] = ts; tau[1] = ta; tau[2] = tb;
+    energy = uts + uta + utb + kT;  // utxx + vtxy + wtxz + kTx
+  } else if constexpr (DIR == 1) {
+    tau[0] = ta; tau[1] = ts; tau[2] = tb;
+    energy = uta + uts + utb + kT;  // utyx + vtyy + wtyz + kTy
+  } else {
+    tau[0] = ta; tau[1] = tb; tau[2] = ts;
+    energy = uta + utb + uts + kT;  // utzx + vtzy + wtzz + kTz
+  }
+}
+
+// Cross derivative stencils of the LDS loaders
+// (3D_solver/src/load_smem_visc_cent.f90.fypp:52-66): f[m] = value at offset
+// m - VO/2 ... (centre excluded is never read).
+template <class R, int VO> __device__ __forceinline__ R cross_d(const PhysC<R>& c, R inv, const R* f) {
+  // f has 2*(VO/2)+1 entries centred on index VO/2 (VO=4: 5 entries, VO=6: 7)
+  if constexpr (VO == 4) {
+    return (c.two_third * (-f[1] + f[3]) - c.one_twelfth * (-f[0] + f[4])) * inv;
+  } else {
+    return (c.one_sixty * (-f[0] + f[6]) + R(0.15) * (f[1] - f[5]) + R(0.75) * (-f[2] + f[4])) * inv;
+  }
+}
+
+}  // namespace ox

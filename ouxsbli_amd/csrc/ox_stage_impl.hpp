@@ -1,0 +1,65 @@
+// ox_stage_impl.hpp -- stage launcher templates and the specialisation table.
+//
+// The reference specialises each case at build time with fypp
+// (src/mod_constant.f90.fypp:37-100: VISC, ORDER, SCHEME encoded as kinds);
+// here (ORDER, SCHEME, VISC, VISC_ORDER) are template parameters and every
+// supported combination is instantiated once.  TVD kind and SLAU variant are
+// uniform runtime switches.
+#pragma once
+
+#include "ox_core.hpp"
+#include "ox_kernels.hpp"
+
+namespace ox {
+
+inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
+void run_stage(Core& C, const double* Qsrc, const StageArgs& sa, double cx, double cy, double cz) {
+  const Geom& g = C.g;
+  Prim<S> P;
+  P.rho = (S*)C.prim[0];
+  P.u = (S*)C.prim[1];
+  P.v = (S*)C.prim[2];
+  P.w = (S*)C.prim[3];
+  P.p = (S*)C.prim[4];
+  P.T = (S*)C.prim[5];
+  P.mu = (S*)C.prim[6];
+  P.s = (S*)C.prim[7];
+  const PhysC<double> c64 = make_physc<double>(C.hc);
+  const PhysC<R> cr = make_physc<R>(C.hc);
+  const dim3 blk(64, 4, 1);
+  k_prim<S, (VISC != 0)><<<dim3(cdiv(g.nx, 64), cdiv(g.ny, 4), g.nz), blk, 0, C.stream>>>(Qsrc, P, g, c64);
+  if constexpr (SCHEME == SLAU && ORDER > 2)
+    k_sensor<S, R><<<dim3(cdiv(g.nx - 2, 64), cdiv(g.ny - 2, 4), g.nz - 2), blk, 0, C.stream>>>(P, g);
+  k_sweep_x<R, S, ORDER, SCHEME, VISC, VO>
+      <<<dim3(cdiv(g.ix, XT), cdiv(g.iy, XROWS), g.iz), blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd, C.variant, cx);
+  k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 1>
+      <<<dim3(cdiv(g.ix, 64), cdiv(g.iz, 4), cdiv(g.iy, LSEG)), blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd,
+                                                                                    C.variant, cy, sa);
+  k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 2>
+      <<<dim3(cdiv(g.ix, 64), cdiv(g.iy, 4), cdiv(g.iz, LSEG)), blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd,
+                                                                                    C.variant, cz, sa);
+  if (sa.ep != EP_RHS) launch_ghost_fill(C, sa.Qout);
+}
+
+// Supported (ORDER, SCHEME, VISC, VO) combinations.  VO is ignored for
+// Euler (pass 0); VO=6 needs ORDER=6 (three ghost layers).
+#define OX_STAGE_TABLE(R, S)                                                  \
+  switch (order * 1000 + scheme * 100 + visc * 10 + (visc ? vorder : 0)) {    \
+    case 2000: return &run_stage<R, S, 2, KEEP, 0, 0>;                        \
+    case 4000: return &run_stage<R, S, 4, KEEP, 0, 0>;                        \
+    case 6000: return &run_stage<R, S, 6, KEEP, 0, 0>;                        \
+    case 4014: return &run_stage<R, S, 4, KEEP, 1, 4>;                        \
+    case 6014: return &run_stage<R, S, 6, KEEP, 1, 4>;                        \
+    case 6016: return &run_stage<R, S, 6, KEEP, 1, 6>;                        \
+    case 2100: return &run_stage<R, S, 2, SLAU, 0, 0>;                        \
+    case 4100: return &run_stage<R, S, 4, SLAU, 0, 0>;                        \
+    case 6100: return &run_stage<R, S, 6, SLAU, 0, 0>;                        \
+    case 4114: return &run_stage<R, S, 4, SLAU, 1, 4>;                        \
+    case 6114: return &run_stage<R, S, 6, SLAU, 1, 4>;                        \
+    case 6116: return &run_stage<R, S, 6, SLAU, 1, 6>;                        \
+    default: return nullptr;                                                  \
+  }
+
+}  // namespace ox

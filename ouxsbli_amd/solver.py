@@ -1,0 +1,105 @@
+"""Python handle over one device-resident solve (the C ABI of include/ouxsbli.h).
+
+``Solver`` replaces what the reference's ``RungeKutta`` does inside its
+executable (3D_solver/src/calc_time_dev.f90.fypp:30-114, :224-300): it owns
+the device state, advances it by RK steps and hands back the conserved
+fields.  All arithmetic runs in ``libouxsbli.so`` on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _abi, _native
+from .cases import CaseSpec
+
+
+def _dp(a: np.ndarray):
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise ValueError("expected a C-contiguous float64 array")
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Solver:
+    def __init__(self, spec: CaseSpec):
+        self.spec = spec
+        self._lib = _native.load()
+        self._cfg = spec.to_ox()
+        h = ctypes.c_void_p()
+        _native.check(self._lib.ox_create(ctypes.byref(self._cfg), ctypes.byref(h)))
+        self._h = h
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.ox_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state -------------------------------------------------------------
+    def _check_shape(self, a: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != self.spec.state_shape():
+            raise ValueError(f"state shape {a.shape} != {self.spec.state_shape()}")
+        return a
+
+    def set_state(self, Q: np.ndarray) -> None:
+        """Physical conserved state Q(nx,5,ny,nz) (numpy shape (nz, ny, 5, nx))."""
+        Q = self._check_shape(Q)
+        _native.check(self._lib.ox_set_state(self._h, _dp(Q)))
+
+    def get_state(self) -> np.ndarray:
+        Q = np.empty(self.spec.state_shape())
+        _native.check(self._lib.ox_get_state(self._h, _dp(Q)))
+        return Q
+
+    def set_qj(self, QJ: np.ndarray) -> None:
+        QJ = self._check_shape(QJ)
+        _native.check(self._lib.ox_set_qj(self._h, _dp(QJ)))
+
+    def get_qj(self) -> np.ndarray:
+        QJ = np.empty(self.spec.state_shape())
+        _native.check(self._lib.ox_get_qj(self._h, _dp(QJ)))
+        return QJ
+
+    # -- compute -----------------------------------------------------------
+    def step(self, n: int = 1) -> float:
+        """Advance ``n`` RK steps; returns the device time in milliseconds."""
+        _native.check(self._lib.ox_step(self._h, int(n)))
+        ms = ctypes.c_double()
+        _native.check(self._lib.ox_last_step_ms(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def rhs(self) -> np.ndarray:
+        R = np.empty(self.spec.state_shape())
+        _native.check(self._lib.ox_rhs(self._h, _dp(R)))
+        return R
+
+    def diagnostics(self) -> dict:
+        d = _abi.OxDiag()
+        _native.check(self._lib.ox_diagnostics(self._h, ctypes.byref(d)))
+        return d.as_dict()
+
+    def check_state(self):
+        """(ok, first_bad_linear_index, count)."""
+        idx = ctypes.c_int64()
+        cnt = ctypes.c_int64()
+        rc = self._lib.ox_check_state(self._h, ctypes.byref(idx), ctypes.byref(cnt))
+        if rc not in (_abi.OX_OK, _abi.OX_ESTATE):
+            _native.check(rc)
+        return rc == _abi.OX_OK, idx.value, cnt.value
+
+    def synchronize(self) -> None:
+        _native.check(self._lib.ox_synchronize(self._h))

@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark: Mcell-updates/s per RK stage of the RHS + Runge-Kutta path.
+
+Workload (one GPU): supersonic NS Taylor-Green vortex, 256^3 periodic
+interior (262^3 with ghosts), Ma 1.25, Re 1600, SLAU ORDER 6 with HR-SLAU2 and
+hybrid TVD (threshold 0.4), central-6 viscous terms, TVD-RK3, CFL 0.03
+(synthetic initial field, no I/O).  One "step" is one full RK3 step (three
+stages).  `value` is the FP64 plain form; the FP32-bulk form of the same
+workload is measured beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+"""
+import argparse
+import ctypes
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def _opcounts():
+    rows = [json.loads(l) for l in (ROOT / "ouxsbli_amd" / "flops" / "opcount.json").read_text().splitlines() if l]
+    return {r["config"]: r for r in rows}
+
+
+def kernel_model(spec, ops):
+    """Compulsory HBM bytes and algorithmic FP ops per launch of each kernel of
+    the stage pipeline (FP64 element = 8 B).  Bytes count every array element a
+    kernel must read or write once; ops come from ouxsbli_amd/flops/opcount.json."""
+    padded = spec.nx * spec.ny * spec.nz
+    cells = spec.interior_cells
+    e = 8 if spec.precision == "fp64" else 4
+    nprim = 8  # rho,u,v,w,p,T,mu,sensor
+    face = ops["conv_face_smooth"] + 2 + ops["visc_face"] + 4 * ops["cross_deriv"]
+    m = {
+        "prim": {"bytes": padded * (40 + 7 * e), "ops": padded * 20},
+        "sensor": {"bytes": padded * (3 * e + e), "ops": padded * 39},
+        "sweep_x": {"bytes": cells * (nprim * e + 40), "ops": cells * (face + 10)},
+        "sweep_y": {"bytes": cells * (nprim * e + 80), "ops": cells * (face + 15)},
+        # average over the three RK3 stages: stage 1 reads Qin, writes Qout;
+        # stages 2-3 also read Qout
+        "sweep_z_rk": {"bytes": cells * (nprim * e + 40 + 40 + 40 + 40 * 2 / 3), "ops": cells * (face + 15 + 4 * 5)},
+        "ghost_fill": {"bytes": (padded - cells) * 80, "ops": 0},
+    }
+    return m
+
+
+def cpu_baseline(spec_small, seconds=15.0):
+    """The FP64 CPU oracle on a bounded sample of the same scheme."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from ouxsbli_amd import cases
+    import oracle_lib
+    o = oracle_lib.Oracle(spec_small)
+    o.set_state(cases.tgv_state(spec_small))
+    o.step(1)  # warm-up
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        o.step(1)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 500:
+            break
+    rate = spec_small.interior_cells * spec_small.stages * n / el / 1e6
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    return {"value": rate, "unit": "Mcell-updates/s per RK stage", "cores": cores, "kind": "port",
+            "sample": f"oracle (oracle/liboxoracle.so, FP64 OpenMP) on the same scheme at "
+                      f"{spec_small.interior[0]}^3 interior, 1 warm-up + {n} timed RK3 steps ({el:.1f} s)"}
+
+
+def run_form(spec, steps, warmup, profile):
+    from ouxsbli_amd import cases
+    from ouxsbli_amd.solver import Solver
+    Q = cases.tgv_state(spec)
+    with Solver(spec) as s:
+        s.set_state(Q)
+        if warmup:
+            s.step(warmup)
+        s.set_profiling(profile)
+        s.synchronize()
+        t0 = time.perf_counter()
+        dev_ms = s.step(steps)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        kt = s.kernel_times() if profile else {}
+        diag = s.diagnostics()
+        ok, _, _ = s.check_state()
+    return {"wall_s": wall, "dev_ms": dev_ms, "kernels": kt, "diag": diag, "ok": ok}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no multi-GPU path; "
+              "the z-slab sharding design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    from ouxsbli_amd import cases
+    spec = cases.config2_spec(args.n, "fp64")
+    spec32 = cases.config2_spec(args.n, "fp32_bulk")
+    ops = _opcounts()["slau6_hrslau2_hybrid_ns6_rk3"]
+
+    r64 = run_form(spec, args.steps, args.warmup, True)
+    r32 = run_form(spec32, args.steps, args.warmup, True)
+
+    cell_stages = spec.interior_cells * spec.stages * args.steps
+    value = cell_stages / r64["wall_s"] / 1e6
+    value32 = cell_stages / r32["wall_s"] / 1e6
+    ke64, ke32 = r64["diag"]["ke"], r32["diag"]["ke"]
+
+    # dominant kernel of the FP64 form, measured live with HIP events
+    model = kernel_model(spec, ops)
+    kt = r64["kernels"]
+    dom = max(kt, key=lambda k: kt[k][0])
+    avg_ms = kt[dom][0] / kt[dom][1]
+    achieved = model[dom]["bytes"] / (avg_ms * 1e-3) / 1e9
+    kernels = {}
+    for k, (tot, n) in kt.items():
+        a = tot / n
+        kernels[k] = {"avg_ms": round(a, 4), "share": round(tot / sum(v[0] for v in kt.values()), 4),
+                      "alg_GBs": round(model[k]["bytes"] / (a * 1e-3) / 1e9, 1),
+                      "TFLOPs": round(model[k]["ops"] / (a * 1e-3) / 1e12, 2)}
+    F_alg = ops["per_cell_stage_smooth"]
+    B_alg = 320.0 / 3.0
+    stage_rate = value * 1e6
+    bound_rate = min(HBM_PEAK_GBS * 1e9 / B_alg, FP64_PEAK_TFLOPS * 1e12 / F_alg)
+
+    line = {
+        "metric": "Mcell-updates/s per RK stage, FP32-bulk vs FP64 plain; p50/p99 solve latency",
+        "value": round(value, 1),
+        "unit": "Mcell-updates/s per RK stage",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r64["wall_s"] * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"supersonic NS TGV {args.n}^3 periodic, Ma 1.25, Re 1600, SLAU ORDER 6 HR-SLAU2 "
+                               "hybrid-TVD (threshold 0.4), central-6 viscous, TVD-RK3, CFL 0.03",
+                   "interior": [args.n] * 3, "extents": [spec.nx, spec.ny, spec.nz], "stages_per_step": 3,
+                   "form": "FP64 plain (bit-identical to the CPU oracle)"},
+        "fp32_bulk": {"value": round(value32, 1), "ms_per_step": round(r32["wall_s"] * 1e3 / args.steps, 3),
+                      "speedup_vs_fp64": round(value32 / value, 3),
+                      "ke_rel_err_vs_fp64": abs(ke32 - ke64) / abs(ke64)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "alg_bytes_per_launch": model[dom]["bytes"], "avg_launch_ms": round(avg_ms, 4)},
+        "stage_roofline": {"F_alg_per_cell_stage": F_alg, "B_alg_per_cell_stage": round(B_alg, 1),
+                           "binding": "fp64" if FP64_PEAK_TFLOPS * 1e12 / F_alg < HBM_PEAK_GBS * 1e9 / B_alg else "hbm",
+                           "bound_Mcell_stage_s": round(bound_rate / 1e6, 1),
+                           "frac": round(stage_rate / bound_rate, 4)},
+        "kernels": kernels,
+        "state_ok": bool(r64["ok"] and r32["ok"]),
+    }
+    if not args.no_cpu:
+        small = cases.config2_spec(64, "fp64")
+        line["cpu_baseline"] = cpu_baseline(small, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

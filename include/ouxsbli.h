@@ -163,6 +163,20 @@ int ox_synchronize(ox_solver* s);
  * handle's stream). */
 int ox_last_step_ms(ox_solver* s, double* ms);
 
+/* Per-kernel device time, accumulated while profiling is enabled. */
+typedef struct ox_kernel_time {
+  char name[32];      /* "prim", "sensor", "sweep_x", "sweep_y", "sweep_z_rk", "ghost_fill" */
+  double total_ms;    /* sum of launch durations (HIP events on the handle's stream) */
+  int64_t launches;
+} ox_kernel_time;
+
+/* Instrumentation (no counterpart in the reference, which relies on external
+ * nsys/ncu runs, 3D_solver/NSTGV/profile_ncu.sh): while enabled, ox_step
+ * records a HIP event after every kernel on the handle's stream and
+ * accumulates per-kernel durations.  Enabling clears the accumulators. */
+int ox_set_profiling(ox_solver* s, int32_t enable);
+int ox_kernel_times(ox_solver* s, ox_kernel_time* out, int32_t cap, int32_t* n);
+
 /* Thread-local text of the last error on this thread ("" if none). */
 const char* ox_last_error(void);
 

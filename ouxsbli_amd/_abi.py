@@ -72,6 +72,16 @@ class OxDiag(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class OxKernelTime(ctypes.Structure):
+    """``ox_kernel_time`` -- accumulated device time of one kernel."""
+
+    _fields_ = [
+        ("name", ctypes.c_char * 32),
+        ("total_ms", ctypes.c_double),
+        ("launches", ctypes.c_int64),
+    ]
+
+
 # Every symbol include/ouxsbli.h declares (checked by tests/test_abi.py).
 EXPORTS = (
     "ox_create",
@@ -86,6 +96,8 @@ EXPORTS = (
     "ox_check_state",
     "ox_synchronize",
     "ox_last_step_ms",
+    "ox_set_profiling",
+    "ox_kernel_times",
     "ox_last_error",
     "ox_version",
 )

@@ -53,6 +53,11 @@ def load() -> ctypes.CDLL:
     L.ox_synchronize.restype = I
     L.ox_last_step_ms.argtypes = [P, D]
     L.ox_last_step_ms.restype = I
+    L.ox_set_profiling.argtypes = [P, ctypes.c_int32]
+    L.ox_set_profiling.restype = I
+    L.ox_kernel_times.argtypes = [P, ctypes.POINTER(_abi.OxKernelTime), ctypes.c_int32,
+                                  ctypes.POINTER(ctypes.c_int32)]
+    L.ox_kernel_times.restype = I
     L.ox_last_error.argtypes = []
     L.ox_last_error.restype = ctypes.c_char_p
     L.ox_version.argtypes = []

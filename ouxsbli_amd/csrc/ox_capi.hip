@@ -71,8 +71,32 @@ __global__ __launch_bounds__(256) void k_scale(double* __restrict__ Q, long long
 
 }  // namespace ox
 
+// Per-kernel event profiling (ox_set_profiling / ox_kernel_times).
+struct ProfState {
+  std::vector<hipEvent_t> pool;       // events recorded this ox_step call
+  std::vector<const char*> names;     // name of the kernel that ends at pool[m] (m >= 1)
+  size_t used = 0;
+  std::vector<std::string> acc_names;
+  std::vector<double> acc_ms;
+  std::vector<long long> acc_n;
+};
+
+static void prof_record(ox::Core* C, const char* name) {
+  ProfState* P = static_cast<ProfState*>(C->prof_state);
+  if (P->used == P->pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    P->pool.push_back(e);
+    P->names.push_back(nullptr);
+  }
+  hipEventRecord(P->pool[P->used], C->stream);
+  P->names[P->used] = name;
+  ++P->used;
+}
+
 struct ox_solver {
   ox::Core C;
+  ProfState prof;
   double* QJ = nullptr;   // state Q^n
   double* QJ2 = nullptr;  // QJ2 (RK3) / QJs (RK4)
   double* Rs = nullptr;   // RK4 accumulator
@@ -152,6 +176,7 @@ void destroy(ox_solver* s) {
   hipFree(s->scratch);
   hipFree(s->C.Racc);
   for (int m = 0; m < 8; ++m) hipFree(s->C.prim[m]);
+  for (hipEvent_t e : s->prof.pool) hipEventDestroy(e);
   if (s->ev0) hipEventDestroy(s->ev0);
   if (s->ev1) hipEventDestroy(s->ev1);
   if (s->C.stream) hipStreamDestroy(s->C.stream);
@@ -217,6 +242,9 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   C.Racc = nullptr;
   for (int m = 0; m < 8; ++m) C.prim[m] = nullptr;
   C.stream = nullptr;
+  C.prof = false;
+  C.mark = &prof_record;
+  C.prof_state = &s->prof;
   if (C.vorder != 0 && C.vorder != 4 && C.vorder != 6) {
     delete s;
     return fail(OX_EINVAL, "visc_order must be 4 or 6");
@@ -321,6 +349,10 @@ int ox_step(ox_solver* s, int32_t nsteps) {
   if (nsteps < 0) return fail(OX_EINVAL, "nsteps < 0");
   OX_HIP(hipSetDevice(s->C.cfg.device));
   OX_HIP(hipEventRecord(s->ev0, s->C.stream));
+  if (s->C.prof) {
+    s->prof.used = 0;
+    prof_record(&s->C, "start");
+  }
   for (int t = 0; t < nsteps; ++t) run_rk_step(s);
   OX_HIP(hipGetLastError());
   OX_HIP(hipEventRecord(s->ev1, s->C.stream));
@@ -328,6 +360,23 @@ int ox_step(ox_solver* s, int32_t nsteps) {
   float ms = 0.f;
   OX_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
   s->last_ms = ms;
+  if (s->C.prof) {
+    ProfState& P = s->prof;
+    for (size_t m = 1; m < P.used; ++m) {
+      float d = 0.f;
+      OX_HIP(hipEventElapsedTime(&d, P.pool[m - 1], P.pool[m]));
+      size_t a = 0;
+      while (a < P.acc_names.size() && P.acc_names[a] != P.names[m]) ++a;
+      if (a == P.acc_names.size()) {
+        P.acc_names.emplace_back(P.names[m]);
+        P.acc_ms.push_back(0.0);
+        P.acc_n.push_back(0);
+      }
+      P.acc_ms[a] += d;
+      P.acc_n[a] += 1;
+    }
+    P.used = 0;
+  }
   return OX_OK;
 }
 
@@ -414,6 +463,29 @@ int ox_synchronize(ox_solver* s) {
   if (!s) return fail(OX_EINVAL, "null handle");
   OX_HIP(hipSetDevice(s->C.cfg.device));
   OX_HIP(hipStreamSynchronize(s->C.stream));
+  return OX_OK;
+}
+
+int ox_set_profiling(ox_solver* s, int32_t enable) {
+  if (!s) return fail(OX_EINVAL, "null handle");
+  s->C.prof = enable != 0;
+  s->prof.acc_names.clear();
+  s->prof.acc_ms.clear();
+  s->prof.acc_n.clear();
+  s->prof.used = 0;
+  return OX_OK;
+}
+
+int ox_kernel_times(ox_solver* s, ox_kernel_time* out, int32_t cap, int32_t* n) {
+  if (!s || !n) return fail(OX_EINVAL, "null argument");
+  const ProfState& P = s->prof;
+  *n = (int32_t)P.acc_names.size();
+  for (int32_t a = 0; a < *n && a < cap && out; ++a) {
+    std::memset(out[a].name, 0, sizeof(out[a].name));
+    std::strncpy(out[a].name, P.acc_names[a].c_str(), sizeof(out[a].name) - 1);
+    out[a].total_ms = P.acc_ms[a];
+    out[a].launches = P.acc_n[a];
+  }
   return OX_OK;
 }
 

@@ -19,7 +19,15 @@ struct Core {
   void* prim[8];     // rho,u,v,w,p,T,mu,sensor
   double* Racc;
   hipStream_t stream;
+  // profiling: when `prof` is set, mark() records an event after a kernel
+  bool prof;
+  void (*mark)(Core* C, const char* name);
+  void* prof_state;
 };
+
+inline void prof_mark(Core& C, const char* name) {
+  if (C.prof) C.mark(&C, name);
+}
 
 // Launches one right-hand-side evaluation on state Qsrc followed by the
 // stage epilogue `sa` (ghost fill of sa.Qout included unless EP_RHS).

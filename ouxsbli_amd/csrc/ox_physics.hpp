@@ -14,17 +14,23 @@
 
 #include "ox_types.hpp"
 
+// Device functions are also host-callable so that the operation-counting
+// instantiation (ouxsbli_amd/flops/) can run them on the CPU.
+#ifndef OX_HD
+#define OX_HD __host__ __device__ __forceinline__
+#endif
+
 namespace ox {
 
 // Fortran MIN/MAX as a select: the first operand wins ties, so the sign of a
 // zero result is well defined.
-template <class R> __device__ __forceinline__ R fmin2(R a, R b) { return (b < a) ? b : a; }
-template <class R> __device__ __forceinline__ R fmax2(R a, R b) { return (b > a) ? b : a; }
+template <class R> OX_HD R fmin2(R a, R b) { return (b < a) ? b : a; }
+template <class R> OX_HD R fmax2(R a, R b) { return (b > a) ? b : a; }
 // SIGN(1, x) with IEEE signed zero.
-template <class R> __device__ __forceinline__ R fsign1(R x) { return __builtin_copysign(R(1), x); }
-template <class R> __device__ __forceinline__ R fabs_(R x) { return __builtin_fabs(x); }
-__device__ __forceinline__ double fsqrt(double x) { return __builtin_sqrt(x); }
-__device__ __forceinline__ float fsqrt(float x) { return __builtin_sqrtf(x); }
+template <class R> OX_HD R fsign1(R x) { return __builtin_copysign(R(1), x); }
+template <class R> OX_HD R fabs_(R x) { return __builtin_fabs(x); }
+OX_HD double fsqrt(double x) { return __builtin_sqrt(x); }
+OX_HD float fsqrt(float x) { return __builtin_sqrtf(x); }
 
 // Per-case constants in the arithmetic type R (src/mod_constant.f90.fypp:18-35).
 template <class R> struct PhysC {
@@ -53,7 +59,7 @@ template <class R> __host__ PhysC<R> make_physc(const HostConsts& h) {
 // signed zeros match the reference expression).
 // ---------------------------------------------------------------------------
 template <class R, int DIR>
-__device__ __forceinline__ void keep2(const PhysC<R>& c, const R* rho, const R* u, const R* v, const R* w,
+OX_HD void keep2(const PhysC<R>& c, const R* rho, const R* u, const R* v, const R* w,
                                       const R* p, const R* T, R* F) {
   const R* uu = DIR == 0 ? u : (DIR == 1 ? v : w);
   const R N1 = DIR == 0 ? R(1) : R(0), N2 = DIR == 1 ? R(1) : R(0), N3 = DIR == 2 ? R(1) : R(0);
@@ -67,7 +73,7 @@ __device__ __forceinline__ void keep2(const PhysC<R>& c, const R* rho, const R* 
 }
 
 template <class R, int DIR>
-__device__ __forceinline__ void keep4(const PhysC<R>& c, const R* rho, const R* u, const R* v, const R* w,
+OX_HD void keep4(const PhysC<R>& c, const R* rho, const R* u, const R* v, const R* w,
                                       const R* p, const R* T, R* F) {
   const R* uu = DIR == 0 ? u : (DIR == 1 ? v : w);
   const R N1 = DIR == 0 ? R(1) : R(0), N2 = DIR == 1 ? R(1) : R(0), N3 = DIR == 2 ? R(1) : R(0);
@@ -92,7 +98,7 @@ __device__ __forceinline__ void keep4(const PhysC<R>& c, const R* rho, const R* 
 }
 
 template <class R, int DIR>
-__device__ __forceinline__ void keep6(const PhysC<R>& c, const R* rho, const R* u, const R* v, const R* w,
+OX_HD void keep6(const PhysC<R>& c, const R* rho, const R* u, const R* v, const R* w,
                                       const R* p, const R* T, R* F) {
   const R* uu = DIR == 0 ? u : (DIR == 1 ? v : w);
   const R N1 = DIR == 0 ? R(1) : R(0), N2 = DIR == 1 ? R(1) : R(0), N3 = DIR == 2 ? R(1) : R(0);
@@ -133,22 +139,22 @@ __device__ __forceinline__ void keep6(const PhysC<R>& c, const R* rho, const R* 
 // ---------------------------------------------------------------------------
 // MUSCL.  src/calc_muscl.f90:19-169.  tvd: OX_TVD_NONE / TVD / HYBRID.
 // ---------------------------------------------------------------------------
-template <class R> __device__ __forceinline__ R minmod2(R x, R y) {
+template <class R> OX_HD R minmod2(R x, R y) {
   const R sgn = fsign1(x);
   return sgn * fmax2(fmin2(fabs_(x), sgn * y), R(0));
 }
-template <class R> __device__ __forceinline__ R minmod3(R x, R y, R z) {
+template <class R> OX_HD R minmod3(R x, R y, R z) {
   const R sgn = fsign1(x);
   return sgn * fmax2(fmin2(fmin2(fabs_(x), sgn * y), sgn * z), R(0));
 }
-template <class R> __device__ __forceinline__ R d33(R d1, R d2, R d3) {
+template <class R> OX_HD R d33(R d1, R d2, R d3) {
   return minmod3(d1, R(2) * d2, R(2) * d3) - R(2) * minmod3(d2, R(2) * d1, R(2) * d3) +
          minmod3(d3, R(2) * d1, R(2) * d2);
 }
 
 // delta4 -> MUSCL3rd{nonTVD,Minmod,Threshold} (:44-91, :142-153)
 template <class R>
-__device__ __forceinline__ void delta4(const PhysC<R>& c, int tvd, R sensor, const R* a, R& al, R& ar) {
+OX_HD void delta4(const PhysC<R>& c, int tvd, R sensor, const R* a, R& al, R& ar) {
   const R d1 = -a[0] + a[1];
   const R d2 = -a[1] + a[2];
   const R d3 = -a[2] + a[3];
@@ -168,7 +174,7 @@ __device__ __forceinline__ void delta4(const PhysC<R>& c, int tvd, R sensor, con
 
 // delta6 -> MUSCL4th{nonTVD,TVD,Threshold} (:94-139, :156-169)
 template <class R>
-__device__ __forceinline__ void delta6(const PhysC<R>& c, int tvd, R sensor, const R* a, R& al, R& ar) {
+OX_HD void delta6(const PhysC<R>& c, int tvd, R sensor, const R* a, R& al, R& ar) {
   const R d1 = -a[0] + a[1];
   const R d2 = -a[1] + a[2];
   const R d3 = -a[2] + a[3];
@@ -195,7 +201,7 @@ __device__ __forceinline__ void delta6(const PhysC<R>& c, int tvd, R sensor, con
 }
 
 // 3D_solver/src/calc_hybrid.f90:75-81
-template <class R> __device__ __forceinline__ R wiggle(const R* phi) {
+template <class R> OX_HD R wiggle(const R* phi) {
   const R phi1 = (-phi[0] + phi[1]) * (-phi[1] + phi[2]);
   const R phi2 = (-phi[2] + phi[3]) * (-phi[1] + phi[2]);
   return R(0.5) * (R(1) - fsign1(fmin2(phi1, phi2)));
@@ -205,7 +211,7 @@ template <class R> __device__ __forceinline__ R wiggle(const R* phi) {
 // SLAU1 / HRSLAU2.  3D_solver/src/calc_slau_3d.f90:1-104.
 // ---------------------------------------------------------------------------
 template <class R, int DIR>
-__device__ __forceinline__ void slau(const PhysC<R>& c, int variant, R rho1, R rho2, R u1, R u2, R v1, R v2, R w1,
+OX_HD void slau(const PhysC<R>& c, int variant, R rho1, R rho2, R u1, R u2, R v1, R v2, R w1,
                                      R w2, R p1, R p2, R HR, R* F) {
   const R un1 = DIR == 0 ? u1 : (DIR == 1 ? v1 : w1);
   const R un2 = DIR == 0 ? u2 : (DIR == 1 ? v2 : w2);
@@ -257,7 +263,7 @@ __device__ __forceinline__ void slau(const PhysC<R>& c, int variant, R rho1, R r
 // SLAU face: interp2/4/6 (calc_slau_kernel_internal.f90.fypp:31-81) then SLAU.
 // N = stencil length (2, 4, 6); fsens = face Ducros sensor.
 template <class R, int DIR, int N>
-__device__ __forceinline__ void slau_face(const PhysC<R>& c, int tvd, int variant, const R* rho, const R* u,
+OX_HD void slau_face(const PhysC<R>& c, int tvd, int variant, const R* rho, const R* u,
                                           const R* v, const R* w, const R* p, R fsens, R* F) {
   R rl, rr, ul, ur, vl, vr, wl, wr, pl, pr, hr;
   if constexpr (N == 2) {
@@ -290,11 +296,11 @@ __device__ __forceinline__ void slau_face(const PhysC<R>& c, int tvd, int varian
 //   DIR 2 (G): ux, wx, vy, wy
 // Returns tau[3] (subtracted from components 2..4) and the energy term.
 // ---------------------------------------------------------------------------
-template <class R, int N> __device__ __forceinline__ R vinterp(const R* a) {
+template <class R, int N> OX_HD R vinterp(const R* a) {
   if constexpr (N == 4) return R(0.0625) * (-a[0] + R(9) * (a[1] + a[2]) - a[3]);
   else return R(0.00390625) * (R(3) * a[0] - R(25) * a[1] + R(150) * (a[2] + a[3]) - R(25) * a[4] + R(3) * a[5]);
 }
-template <class R, int N> __device__ __forceinline__ R vdiff(const PhysC<R>& c, R d, const R* a) {
+template <class R, int N> OX_HD R vdiff(const PhysC<R>& c, R d, const R* a) {
   if constexpr (N == 4) return (a[0] - R(27) * (a[1] - a[2]) - a[3]) * c.one_24 * d;
   else
     return (R(-0.0046875) * a[0] + R(0.065104167) * a[1] - R(1.171875) * (a[2] - a[3]) - R(0.065104167) * a[4] +
@@ -303,7 +309,7 @@ template <class R, int N> __device__ __forceinline__ R vdiff(const PhysC<R>& c, 
 }
 
 template <class R, int DIR, int N>
-__device__ __forceinline__ void visc_face(const PhysC<R>& c, R d, const R* mu, const R* T, const R* u, const R* v,
+OX_HD void visc_face(const PhysC<R>& c, R d, const R* mu, const R* T, const R* u, const R* v,
                                           const R* w, const R* d1, const R* d2, const R* d3, const R* d4, R* tau,
                                           R& energy) {
   const R mui = vinterp<R, N>(mu);
@@ -341,7 +347,7 @@ __device__ __forceinline__ void visc_face(const PhysC<R>& c, R d, const R* mu, c
 // Cross derivative stencils of the LDS loaders
 // (3D_solver/src/load_smem_visc_cent.f90.fypp:52-66): f[m] = value at offset
 // m - VO/2 ... (centre excluded is never read).
-template <class R, int VO> __device__ __forceinline__ R cross_d(const PhysC<R>& c, R inv, const R* f) {
+template <class R, int VO> OX_HD R cross_d(const PhysC<R>& c, R inv, const R* f) {
   // f has 2*(VO/2)+1 entries centred on index VO/2 (VO=4: 5 entries, VO=6: 7)
   if constexpr (VO == 4) {
     return (c.two_third * (-f[1] + f[3]) - c.one_twelfth * (-f[0] + f[4])) * inv;

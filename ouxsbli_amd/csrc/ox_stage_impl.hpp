@@ -30,17 +30,26 @@ void run_stage(Core& C, const double* Qsrc, const StageArgs& sa, double cx, doub
   const PhysC<R> cr = make_physc<R>(C.hc);
   const dim3 blk(64, 4, 1);
   k_prim<S, (VISC != 0)><<<dim3(cdiv(g.nx, 64), cdiv(g.ny, 4), g.nz), blk, 0, C.stream>>>(Qsrc, P, g, c64);
-  if constexpr (SCHEME == SLAU && ORDER > 2)
+  prof_mark(C, "prim");
+  if constexpr (SCHEME == SLAU && ORDER > 2) {
     k_sensor<S, R><<<dim3(cdiv(g.nx - 2, 64), cdiv(g.ny - 2, 4), g.nz - 2), blk, 0, C.stream>>>(P, g);
+    prof_mark(C, "sensor");
+  }
   k_sweep_x<R, S, ORDER, SCHEME, VISC, VO>
       <<<dim3(cdiv(g.ix, XT), cdiv(g.iy, XROWS), g.iz), blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd, C.variant, cx);
+  prof_mark(C, "sweep_x");
   k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 1>
       <<<dim3(cdiv(g.ix, 64), cdiv(g.iz, 4), cdiv(g.iy, LSEG)), blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd,
                                                                                     C.variant, cy, sa);
+  prof_mark(C, "sweep_y");
   k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 2>
       <<<dim3(cdiv(g.ix, 64), cdiv(g.iy, 4), cdiv(g.iz, LSEG)), blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd,
                                                                                     C.variant, cz, sa);
-  if (sa.ep != EP_RHS) launch_ghost_fill(C, sa.Qout);
+  prof_mark(C, "sweep_z_rk");
+  if (sa.ep != EP_RHS) {
+    launch_ghost_fill(C, sa.Qout);
+    prof_mark(C, "ghost_fill");
+  }
 }
 
 // Supported (ORDER, SCHEME, VISC, VO) combinations.  VO is ignored for

@@ -101,5 +101,16 @@ class Solver:
             _native.check(rc)
         return rc == _abi.OX_OK, idx.value, cnt.value
 
+    def set_profiling(self, enable: bool) -> None:
+        _native.check(self._lib.ox_set_profiling(self._h, 1 if enable else 0))
+
+    def kernel_times(self) -> dict:
+        """{kernel name: (total_ms, launches)} accumulated while profiling."""
+        cap = 32
+        arr = (_abi.OxKernelTime * cap)()
+        n = ctypes.c_int32()
+        _native.check(self._lib.ox_kernel_times(self._h, arr, cap, ctypes.byref(n)))
+        return {arr[a].name.decode(): (arr[a].total_ms, arr[a].launches) for a in range(min(n.value, cap))}
+
     def synchronize(self) -> None:
         _native.check(self._lib.ox_synchronize(self._h))

@@ -1,0 +1,286 @@
+"""Pins of the FP64 CPU oracle (oracle/) by closed forms, invariants, the
+reference's own test thresholds and brute force -- chosen so that a dropped
+term, a wrong sign or index, or a transposed operand fails one of them."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from ouxsbli_amd import cases, _abi
+from oracle_lib import Oracle, lib
+
+
+def _cfg(**kw):
+    s = cases.CaseSpec(nx=16, ny=16, nz=16, **kw)
+    return s.to_ox()
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _euler_flux(rho, u, v, w, p, gamma, d):
+    un = (u, v, w)[d]
+    E = p / (gamma - 1) + 0.5 * rho * (u * u + v * v + w * w)
+    F = np.array([rho * un, rho * un * u, rho * un * v, rho * un * w, un * (E + p)])
+    F[1 + d] += p
+    return F
+
+
+# ---------------------------------------------------------------- fluxes ---
+@pytest.mark.parametrize("order", [2, 4, 6])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_keep_consistency(order, d):
+    """KEEP F(q,...,q) = physical Euler flux (calc_keep_3d.f90:1-111)."""
+    cfg = _cfg()
+    rho, u, v, w, p = 1.3, 0.7, -0.4, 0.25, 2.1e5
+    T = p / (rho * cfg.R)
+    st = np.zeros((6, 6))
+    for m, val in enumerate([rho, u, v, w, p, T]):
+        st[m, :] = val
+    out = np.zeros(5)
+    lib().oxo_keep_flux(ctypes.byref(cfg), order, _dp(st), d, _dp(out))
+    np.testing.assert_allclose(out, _euler_flux(rho, u, v, w, p, cfg.gamma, d), rtol=1e-13)
+
+
+@pytest.mark.parametrize("order", [2, 4, 6])
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("d", [0, 1, 2])
+@pytest.mark.parametrize("un", [0.3, -0.3, 1.7, -1.7])
+def test_slau_consistency(order, variant, d, un):
+    """SLAU1 / HR-SLAU2 F(q,q) = Euler flux, subsonic and supersonic
+    (calc_slau_3d.f90:46-104)."""
+    cfg = _cfg()
+    rho, p = 1.1, 1.0e5
+    c = math.sqrt(cfg.gamma * p / rho)
+    vel = [0.2 * c, -0.1 * c, 0.15 * c]
+    vel[d] = un * c
+    st = np.zeros((6, 6))
+    for m, val in enumerate([rho, *vel, p, p / (rho * cfg.R)]):
+        st[m, :] = val
+    out = np.zeros(5)
+    for tvd in (0, 1, 2):
+        cfg.tvd = tvd
+        cfg.slau_variant = variant
+        lib().oxo_slau_flux(ctypes.byref(cfg), order, _dp(st), d, 0.9, _dp(out))
+        np.testing.assert_allclose(out, _euler_flux(rho, *vel, p, cfg.gamma, d), rtol=1e-12, atol=1e-9)
+
+
+def test_slau_mirror_symmetry():
+    """Mirroring the stencil and the normal velocity flips the mass flux."""
+    cfg = _cfg()
+    cfg.slau_variant = 1
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        st = np.zeros((6, 6))
+        st[0] = 1 + 0.2 * rng.random(6)
+        st[1:4] = 50 * rng.standard_normal((3, 6))
+        st[4] = 1e5 * (1 + 0.2 * rng.random(6))
+        mir = st[:, ::-1].copy()
+        mir[1] = -mir[1]
+        a, b = np.zeros(5), np.zeros(5)
+        lib().oxo_slau_flux(ctypes.byref(cfg), 6, _dp(st), 0, 0.1, _dp(a))
+        lib().oxo_slau_flux(ctypes.byref(cfg), 6, _dp(mir), 0, 0.1, _dp(b))
+        np.testing.assert_allclose(a[0], -b[0], rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(a[1], b[1], rtol=1e-12)  # momentum flux: even
+        np.testing.assert_allclose(a[4], -b[4], rtol=1e-12, atol=1e-9)
+
+
+# ----------------------------------------------------------------- MUSCL ---
+@pytest.mark.parametrize("order", [4, 6])
+def test_muscl_exact_on_linear(order):
+    """Non-TVD and limited reconstructions are exact for linear data: the face
+    value is the midpoint (calc_muscl.f90:44-139)."""
+    cfg = _cfg()
+    a = 3.0 + 0.5 * np.arange(order, dtype=float)
+    face = 0.5 * (a[order // 2 - 1] + a[order // 2])
+    for tvd in (0, 1, 2):
+        al, ar = ctypes.c_double(), ctypes.c_double()
+        lib().oxo_muscl(ctypes.byref(cfg), order, tvd, 0.9, _dp(a), ctypes.byref(al), ctypes.byref(ar))
+        assert al.value == pytest.approx(face, rel=1e-14)
+        assert ar.value == pytest.approx(face, rel=1e-14)
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_muscl_limited_bounds(order):
+    """TVD reconstruction stays within the two adjacent cell values; the
+    hybrid switch picks the limited branch only at sensor >= threshold."""
+    cfg = _cfg(threshold=0.4)
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        a = rng.standard_normal(order)
+        al, ar = ctypes.c_double(), ctypes.c_double()
+        lib().oxo_muscl(ctypes.byref(cfg), order, 1, 0.0, _dp(a), ctypes.byref(al), ctypes.byref(ar))
+        lo = min(a[order // 2 - 1], a[order // 2]) - 1e-12
+        hi = max(a[order // 2 - 1], a[order // 2]) + 1e-12
+        if (a[order // 2] - a[order // 2 - 1]) * (a[order // 2 - 1] - a[order // 2 - 2]) > 0:
+            assert lo <= al.value <= hi
+        al2, ar2 = ctypes.c_double(), ctypes.c_double()
+        lib().oxo_muscl(ctypes.byref(cfg), order, 2, 0.5, _dp(a), ctypes.byref(al2), ctypes.byref(ar2))
+        assert (al2.value, ar2.value) == (al.value, ar.value)
+        lib().oxo_muscl(ctypes.byref(cfg), order, 0, 0.0, _dp(a), ctypes.byref(al), ctypes.byref(ar))
+        lib().oxo_muscl(ctypes.byref(cfg), order, 2, 0.39, _dp(a), ctypes.byref(al2), ctypes.byref(ar2))
+        assert (al2.value, ar2.value) == (al.value, ar.value)
+
+
+def test_wiggle_detector():
+    """calc_hybrid.f90:75-81: monotone -> 0, oscillating -> 1."""
+    assert lib().oxo_wiggle(_dp(np.array([1.0, 2.0, 3.0, 4.0]))) == 0.0
+    assert lib().oxo_wiggle(_dp(np.array([4.0, 3.0, 2.5, 1.0]))) == 0.0
+    assert lib().oxo_wiggle(_dp(np.array([1.0, 2.0, 1.0, 2.0]))) == 1.0
+    assert lib().oxo_wiggle(_dp(np.array([1.0, 2.0, 3.0, 2.5]))) == 1.0
+
+
+# ----------------------------------------------------------- whole RHS ---
+def _small(order=4, scheme="KEEP", visc="NS", n=12, **kw):
+    return cases.tgv_spec(n, order, Ma=0.3, cfl=0.3, cfl_mode="acoustic", scheme=scheme, visc=visc, **kw)
+
+
+@pytest.mark.parametrize("scheme,order,visc", [("KEEP", 4, "NS"), ("KEEP", 6, "NS"), ("SLAU", 6, "NS"),
+                                               ("SLAU", 4, "Euler"), ("KEEP", 2, "Euler")])
+def test_free_stream_preserved(scheme, order, visc):
+    spec = _small(order, scheme, visc)
+    Q = np.zeros(spec.state_shape())
+    Q[:, :, 0, :] = 1.2
+    Q[:, :, 1, :] = 1.2 * 30.0
+    Q[:, :, 2, :] = -1.2 * 20.0
+    Q[:, :, 3, :] = 1.2 * 10.0
+    Q[:, :, 4, :] = 1e5 / 0.4 + 0.5 * 1.2 * (900 + 400 + 100)
+    o = Oracle(spec)
+    o.set_state(Q)
+    assert np.abs(o.rhs()).max() == 0.0
+
+
+@pytest.mark.parametrize("scheme,order,visc,tvd", [("KEEP", 4, "NS", "none"), ("SLAU", 6, "NS", "hybrid"),
+                                                   ("SLAU", 4, "Euler", "tvd")])
+def test_periodic_conservation(scheme, order, visc, tvd):
+    """Sum over the interior of the RHS telescopes to zero (mass, momentum, energy)."""
+    spec = _small(order, scheme, visc, tvd=tvd, Ma=0.9) if False else _small(order, scheme, visc, tvd=tvd)
+    o = Oracle(spec)
+    o.set_state(cases.tgv_state(spec))
+    R = o.rhs()
+    h = spec.order // 2
+    Ri = R[h:-h, h:-h, :, h:-h]
+    for l in range(5):
+        tot = Ri[:, :, l, :].sum()
+        scale = np.abs(Ri[:, :, l, :]).sum() + 1e-300
+        assert abs(tot) <= 1e-12 * scale, (l, tot, scale)
+
+
+def test_ghosts_are_periodic_images_after_step():
+    spec = _small(6, "SLAU", "NS", tvd="hybrid")
+    o = Oracle(spec)
+    o.set_state(cases.tgv_state(spec))
+    o.step(1)
+    QJ = o.get_qj()
+    np.testing.assert_array_equal(QJ, cases.periodic_fill(QJ.copy(), spec.order))
+
+
+def test_tgv_initial_diagnostics():
+    """Interior mean KE of the low-Mach TGV ~ rho0 V0^2/8 (closed form of the
+    initial field); diagnostics agree with brute-force numpy sums."""
+    spec = cases.tgv_spec(24, 4, Ma=0.1)
+    Q = cases.tgv_state(spec)
+    o = Oracle(spec)
+    o.set_state(Q)
+    d = o.diagnostics()
+    assert d["ke"] == pytest.approx(spec.RHO0 * spec.V0 ** 2 / 8.0, rel=2e-3)
+    h = 2
+    Qi = Q[h:-h, h:-h, :, h:-h]
+    rho = Qi[:, :, 0, :]
+    ke = 0.5 * (Qi[:, :, 1, :] ** 2 + Qi[:, :, 2, :] ** 2 + Qi[:, :, 3, :] ** 2) / rho
+    assert d["ke"] == pytest.approx(ke.mean(), rel=1e-12)
+    p = (spec.gamma - 1) * (Qi[:, :, 4, :] - ke)
+    ent = (rho * np.log(p * rho ** (-spec.gamma))).mean()
+    assert d["entropy"] == pytest.approx(ent, rel=1e-10)
+
+
+def _wave_spec(n, order, scheme="KEEP"):
+    """1D density wave advected at uniform u, p: the exact solution translates."""
+    s = cases.CaseSpec(nx=n + order, ny=2 * order + 2, nz=2 * order + 2, order=order, scheme=scheme, visc="Euler",
+                       tvd="none", rk=4, Lx=1.0, Ly=(order + 2) / n, Lz=(order + 2) / n)
+    return s
+
+
+def _wave_state(s, t):
+    h = s.order // 2
+    x = cases.cell_centres(s.nx, s.Lx, s.order)
+    u0, p0 = 1.0, 100.0
+    rho = 1.0 + 0.2 * np.sin(2 * np.pi * (x - u0 * t))
+    Q = np.zeros(s.state_shape())
+    Q[:, :, 0, :] = rho
+    Q[:, :, 1, :] = rho * u0
+    Q[:, :, 4, :] = p0 / (s.gamma - 1) + 0.5 * rho * u0 ** 2
+    return cases.periodic_fill(Q, s.order)
+
+
+@pytest.mark.parametrize("order,min_rate", [(4, 3.6), (6, 5.7)])
+def test_keep_spatial_order(order, min_rate):
+    """Grid convergence of KEEP4 / KEEP6 on a smooth advected wave meets the
+    orders the reference requires of its vortex test (test_evc.py:51-52)."""
+    errs = []
+    ns = [16, 32] if order == 4 else [24, 48]
+    for n in ns:
+        s = _wave_spec(n, order)
+        T = 0.05
+        nsteps = 8 * n
+        s.dt = T / nsteps
+        o = Oracle(s)
+        o.set_state(_wave_state(s, 0.0))
+        o.step(nsteps)
+        h = order // 2
+        err = np.abs(o.get_state()[h:-h, h:-h, 0, h:-h] - _wave_state(s, T)[h:-h, h:-h, 0, h:-h]).max()
+        errs.append(err)
+    rate = math.log(errs[0] / errs[1], 2)
+    assert rate >= min_rate, (errs, rate)
+
+
+@pytest.mark.parametrize("rk,p", [(3, 3), (4, 4)])
+def test_time_order(rk, p):
+    """Temporal convergence: halving dt at fixed T shrinks the error by ~2^p
+    (RK3 Shu-Osher / RK4), measured against a run with dt/16."""
+    s = _wave_spec(16, 4)
+    s.rk = rk
+    T = 0.2
+    sols = {}
+    for m in (8, 16, 128):
+        s.dt = T / m
+        o = Oracle(s)
+        o.set_state(_wave_state(s, 0.0))
+        o.step(m)
+        sols[m] = o.get_state()[:, :, 0, :]
+    e1 = np.abs(sols[8] - sols[128]).max()
+    e2 = np.abs(sols[16] - sols[128]).max()
+    rate = math.log(e1 / e2, 2)
+    assert p - 0.5 <= rate <= p + 0.8, (e1, e2, rate)
+
+
+@pytest.mark.parametrize("vo,order", [(4, 4), (6, 6), (4, 6)])
+def test_viscous_shear_decay_rate(vo, order):
+    """u = U sin(k y) in uniform rho, p: the x-momentum RHS is the viscous
+    term mu*u'' with mu from Sutherland's law (central viscous operator,
+    calc_visc_cent.f90.fypp), converging to -mu k^2 u."""
+    n = 24
+    s = cases.CaseSpec(nx=2 * order + 2, ny=n + order, nz=2 * order + 2, order=order, scheme="KEEP", visc="NS",
+                       visc_order=vo, rk=4, Lx=(order + 2) / n, Ly=1.0, Lz=(order + 2) / n, dt=1e-3)
+    y = cases.cell_centres(s.ny, s.Ly, order)
+    rho, p, U = 1.0, 1.0e5, 2.0
+    k = 2 * np.pi
+    Q = np.zeros(s.state_shape())
+    u = U * np.sin(k * y)
+    Q[:, :, 0, :] = rho
+    Q[:, :, 1, :] = (rho * u)[None, :, None]
+    Q[:, :, 4, :] = (p / (s.gamma - 1) + 0.5 * rho * u ** 2)[None, :, None]
+    Q = cases.periodic_fill(Q, order)
+    o = Oracle(s)
+    o.set_state(Q)
+    R = o.rhs()
+    T = p / (rho * s.R)
+    mu = cases.sutherland_mu(T) if False else 1.716e-5 * 384.2 * 273.2 ** -1.5 / (T + 111.0) * T ** 1.5
+    dx, dy, dz = s.Lx / (s.nx - order), s.Ly / (s.ny - order), s.Lz / (s.nz - order)
+    h = order // 2
+    # R = dt * A * div(F);  dQ/dt * vol = -R/dt  ->  du/dt = -R/(dt*vol*rho)
+    rate = -R[h:-h, h:-h, 1, h:-h] / (s.dt * dx * dy * dz * rho)
+    expected = -mu * k * k * u[h:-h]
+    np.testing.assert_allclose(rate[0, :, 0], expected, rtol=2e-3 if vo == 4 else 1e-4, atol=1e-9 * mu * k * k * U)

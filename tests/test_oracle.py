@@ -244,24 +244,19 @@ def test_time_order(rk, p):
     s.rk = rk
     T = 0.2
     sols = {}
-    for m in (8, 16, 128):
+    for m in (128, 256, 2048):
         s.dt = T / m
         o = Oracle(s)
         o.set_state(_wave_state(s, 0.0))
         o.step(m)
         sols[m] = o.get_state()[:, :, 0, :]
-    e1 = np.abs(sols[8] - sols[128]).max()
-    e2 = np.abs(sols[16] - sols[128]).max()
+    e1 = np.abs(sols[128] - sols[2048]).max()
+    e2 = np.abs(sols[256] - sols[2048]).max()
     rate = math.log(e1 / e2, 2)
     assert p - 0.5 <= rate <= p + 0.8, (e1, e2, rate)
 
 
-@pytest.mark.parametrize("vo,order", [(4, 4), (6, 6), (4, 6)])
-def test_viscous_shear_decay_rate(vo, order):
-    """u = U sin(k y) in uniform rho, p: the x-momentum RHS is the viscous
-    term mu*u'' with mu from Sutherland's law (central viscous operator,
-    calc_visc_cent.f90.fypp), converging to -mu k^2 u."""
-    n = 24
+def _shear_error(n, vo, order):
     s = cases.CaseSpec(nx=2 * order + 2, ny=n + order, nz=2 * order + 2, order=order, scheme="KEEP", visc="NS",
                        visc_order=vo, rk=4, Lx=(order + 2) / n, Ly=1.0, Lz=(order + 2) / n, dt=1e-3)
     y = cases.cell_centres(s.ny, s.Ly, order)
@@ -277,10 +272,24 @@ def test_viscous_shear_decay_rate(vo, order):
     o.set_state(Q)
     R = o.rhs()
     T = p / (rho * s.R)
-    mu = cases.sutherland_mu(T) if False else 1.716e-5 * 384.2 * 273.2 ** -1.5 / (T + 111.0) * T ** 1.5
+    mu = 1.716e-5 * 384.2 * 273.2 ** -1.5 / (T + 111.0) * T ** 1.5  # Sutherland, mod_constant.f90.fypp:26
     dx, dy, dz = s.Lx / (s.nx - order), s.Ly / (s.ny - order), s.Lz / (s.nz - order)
     h = order // 2
-    # R = dt * A * div(F);  dQ/dt * vol = -R/dt  ->  du/dt = -R/(dt*vol*rho)
-    rate = -R[h:-h, h:-h, 1, h:-h] / (s.dt * dx * dy * dz * rho)
-    expected = -mu * k * k * u[h:-h]
-    np.testing.assert_allclose(rate[0, :, 0], expected, rtol=2e-3 if vo == 4 else 1e-4, atol=1e-9 * mu * k * k * U)
+    # R = dt * A * div(F);  d(rho u)/dt * vol = -R/dt.  The scheme differences
+    # face stresses, so the semi-discrete target is mu*(u'(y+dy/2) - u'(y-dy/2))/dy.
+    rate = -R[h:-h, h:-h, 1, h:-h] / (s.dt * dx * dy * dz)
+    yi = y[h:-h]
+    expected = mu * U * k * (np.cos(k * (yi + dy / 2)) - np.cos(k * (yi - dy / 2))) / dy
+    return np.abs(rate[0, :, 0] - expected).max() / np.abs(expected).max()
+
+
+@pytest.mark.parametrize("vo,order,min_rate", [(4, 4, 3.6), (6, 6, 5.6), (4, 6, 3.6)])
+def test_viscous_shear_rate(vo, order, min_rate):
+    """u = U sin(k y) in uniform rho, p: the x-momentum RHS is the viscous term
+    (central viscous operator, calc_visc_cent.f90.fypp).  Its face stress is a
+    VO-th order staggered derivative, so the difference of exact face
+    stresses is matched at that order; mu is Sutherland's law."""
+    e1 = _shear_error(24, vo, order)
+    e2 = _shear_error(48, vo, order)
+    assert e1 < 1e-2
+    assert math.log(e1 / e2, 2) >= min_rate, (e1, e2)

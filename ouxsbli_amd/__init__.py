@@ -5,6 +5,9 @@ The compute path is the in-tree HIP library ``libouxsbli.so`` (C ABI of
 include/ouxsbli.h) loaded with ctypes; there is no CPU fallback.
 """
 from .cases import CaseSpec, config1_spec, config2_spec, tgv_spec, tgv_state  # noqa: F401
+from .case import Case  # noqa: F401
+from .patcher import patch  # noqa: F401
+from .ic import UniformIC, TaylorGreenIC, RiemannIC, PeriodicBC  # noqa: F401
 
 __version__ = "0.1.0"
 

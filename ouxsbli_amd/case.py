@@ -1,0 +1,189 @@
+"""``Case`` -- one simulation run, the drop-in for the reference's
+``ouxsbli.Case`` (ouxsbli/case.py:68-256).
+
+Same constructor, aliases, value normalisation and parameter routing:
+``build()`` copies the case directory into ``workdir`` and patches
+``config.fypp`` / ``mod_globals.f90``; then, where the reference runs
+cmake/make to compile one executable, it reads the patched files into an
+``ox_config`` and checks that the gfx950 library holds that specialisation.
+``run()`` drives the library in-process through ctypes instead of
+``mpirun a.out`` and writes the same run directory: ``data/Q%05d.vtr``,
+``data/kinetic_energy.d``, ``data/entropy.d`` and ``recal/Q00001.dat``.
+"""
+from __future__ import annotations
+
+import json
+import pathlib
+import shutil
+from typing import Any
+
+from . import io_vtr
+from .cases import CaseSpec, cell_centres
+from .fortran_params import read_fypp_macros, read_parameters
+from .ic import bc_to_params, ic_to_params, initial_state
+from .patcher import _is_macro_line, patch
+
+_ALIAS = {
+    "scheme": "SCHEME", "visc": "VISC", "accuracy": "ORDER", "visc_order": "VISC_ORDER", "tvd": "TVD",
+    "slau": "SLAU_VARIANT", "rescale": "RESCALE", "recal": "RESTART", "rk": "RK", "gpumpi": "GPUMPI",
+    "bc_x": "BC_X", "bc_y": "BC_Y", "commz": "COMMZ",
+    # build-specific (no counterpart in the reference): arithmetic of the RHS
+    "precision": "PRECISION", "visc_stencil": "VISC_STENCIL",
+}
+_VALUE_NORMALIZE = {
+    "SCHEME": {"keep": "KEEP", "slau": "SLAU", "roe": "Roe", "hybrid": "Hybrid"},
+    "VISC": {"euler": "Euler", "ns": "NS", "les": "LES"},
+    "TVD": {"none": "none", "tvd": "tvd", "hybrid": "hybrid", "minmod": "tvd", "muscl4": "hybrid"},
+    "SLAU_VARIANT": {"slau": "SLAU", "hrslau2": "HRSLAU2"},
+}
+_RK_NORMALIZE = {"tvd_rk3": 3, "rk3": 3, "rk4": 4, "classical_rk4": 4}
+
+TEMPLATES = pathlib.Path(__file__).resolve().parent / "case_templates"
+
+
+class Case:
+    """Manage one run: copy source -> patch -> specialise -> run on the GPU."""
+
+    def __init__(self, source: str, workdir: str, **params: Any) -> None:
+        src = pathlib.Path(source)
+        self.source = src.resolve() if src.is_absolute() or src.exists() else (TEMPLATES / source).resolve()
+        self.workdir = pathlib.Path(workdir).resolve()
+        self.build_dir = self.workdir / "build"
+        raw = dict(params)
+        ic_spec = raw.pop("ic", None)
+        bc_spec = raw.pop("bc", None)
+        if ic_spec is not None:
+            raw.update(ic_to_params(ic_spec))
+        if bc_spec is not None:
+            raw.update(bc_to_params(bc_spec))
+        self.params = {_ALIAS.get(k.lower(), k): v for k, v in raw.items()}
+        self._built = False
+        self.spec: CaseSpec | None = None
+        self.globals: dict = {}
+        self.macros: dict = {}
+
+    # -- setup: copy + patch (case.py:111-152 of the reference) --------------
+    def _setup(self) -> None:
+        if self.workdir.exists():
+            shutil.rmtree(self.workdir)
+        shutil.copytree(self.source, self.workdir,
+                        ignore=shutil.ignore_patterns("build", "CMakeCache.txt", "CMakeFiles", "*.cmake"))
+        cfg_path = self.workdir / "config.fypp"
+        glob_path = self.workdir / "mod_globals.f90"
+        if not cfg_path.exists():
+            raise FileNotFoundError(f"config.fypp not found in {self.source}")
+        if not glob_path.exists():
+            raise FileNotFoundError(f"mod_globals.f90 not found in {self.source}")
+        cfg_text, glob_text = cfg_path.read_text(), glob_path.read_text()
+        cfg_lines = cfg_text.splitlines()
+        to_cfg, to_glob = {}, {}
+        for k, v in self.params.items():
+            if k == "RK" and isinstance(v, str):
+                v = _RK_NORMALIZE.get(v.lower(), v)
+            if any(_is_macro_line(line, k) for line in cfg_lines):
+                if isinstance(v, str) and k in _VALUE_NORMALIZE:
+                    v = _VALUE_NORMALIZE[k].get(v.lower(), v)
+                to_cfg[k] = v
+            else:
+                to_glob[k] = v
+        cfg_path.write_text(patch(cfg_text, to_cfg))
+        glob_path.write_text(patch(glob_text, to_glob))
+
+    # -- build ---------------------------------------------------------------
+    def build(self) -> None:
+        """Apply patches, read the case into an ox_config and select the
+        prebuilt gfx950 specialisation (raises if none matches)."""
+        self._setup()
+        self.macros = read_fypp_macros((self.workdir / "config.fypp").read_text())
+        self.globals = read_parameters((self.workdir / "mod_globals.f90").read_text())
+        # parameters that were routed to mod_globals but are not declared there
+        # (e.g. IC spec fields) still reach the initial-condition generator
+        for k, v in self.params.items():
+            if k not in self.macros:
+                self.globals.setdefault(k.lower(), v)
+        self.spec = self._make_spec()
+        self.build_dir.mkdir(exist_ok=True)
+        (self.build_dir / "case.json").write_text(json.dumps(
+            {"source": str(self.source), "macros": {k: v for k, v in self.macros.items()},
+             "spec": {k: getattr(self.spec, k) for k in self.spec.__dataclass_fields__}}, indent=1, default=str))
+        self._built = True
+
+    def _make_spec(self) -> CaseSpec:
+        m, g = self.macros, self.globals
+        for flag in ("BC_X", "BC_Y", "BC_Z"):
+            if m.get(flag, False):
+                raise RuntimeError(f"{flag}=True: only periodic boundaries are built into the gfx950 library")
+        if m.get("COMMZ", False):
+            raise RuntimeError("COMMZ=True: one GPU holds the whole grid; z-slab decomposition is not built")
+        if m.get("RESCALE", False):
+            raise RuntimeError("RESCALE=True (SBLI inflow recycling) is not part of this library")
+        order = int(m.get("ORDER", 6))
+        spec = CaseSpec(
+            nx=int(g["nx"]), ny=int(g["ny"]), nz=int(g["nz"]), order=order, scheme=str(m.get("SCHEME", "KEEP")),
+            visc=str(m.get("VISC", "Euler")), visc_order=int(m.get("VISC_ORDER", order)),
+            visc_stencil=str(m.get("VISC_STENCIL", "Central")), tvd=str(m.get("TVD", "none")),
+            slau_variant=str(m.get("SLAU_VARIANT", "HRSLAU2")), rk=int(m.get("RK", 3)),
+            precision=str(m.get("PRECISION", "fp64")), gamma=float(g.get("gamma", 1.4)), R=float(g.get("r", 287.03)),
+            Pr=float(g.get("pr", 0.71)), Prt=float(g.get("prt", 0.9)), threshold=float(g.get("threshold", 0.4)),
+            dt=float(g["dt"]), Lx=float(g["lx"]), Ly=float(g["ly"]), Lz=float(g["lz"]))
+        if spec.visc == "Euler":
+            spec.visc_order = 0
+        return spec
+
+    # -- run -------------------------------------------------------------------
+    def run(self, nranks: int = 2) -> None:
+        """Advance np outputs x nt steps on one GPU, writing the run directory.
+        ``nranks`` is accepted for compatibility (the reference's I/O rank is
+        replaced by in-process writes)."""
+        if not self._built:
+            raise RuntimeError("Must call build() before run()")
+        from .solver import Solver
+        spec = self.spec
+        g = self.globals
+        data = self.workdir / "data"
+        data.mkdir(exist_ok=True)
+        (self.workdir / "recal").mkdir(exist_ok=True)
+        nt, np_ = int(g.get("nt", 1)), int(g.get("np", 1))
+        restart = bool(self.macros.get("RESTART", False))
+        if restart:
+            Q = io_vtr.read_restart(self.workdir / "recal" / "Q00001.dat", spec.state_shape())
+        else:
+            Q = initial_state(self.source.name, spec, g)
+        x = cell_centres(spec.nx, spec.Lx, spec.order)
+        y = cell_centres(spec.ny, spec.Ly, spec.order)
+        z = cell_centres(spec.nz, spec.Lz, spec.order)
+        diag = io_vtr.Diagnostics(data)
+        offset = int(g.get("step_offset", 0))
+        with Solver(spec) as s:
+            s.set_state(Q)
+            jac = self.jacobian()
+
+            def output(step):
+                QJ = s.get_qj()
+                rho, p, vel = io_vtr.primitives_for_print(QJ, jac, spec.gamma)
+                ke, ent = io_vtr.ke_entropy(rho, p, vel, spec.order, spec.gamma)
+                diag.record(step, nt * step * spec.dt, ke, ent)
+                io_vtr.write_vtr(data / f"Q{step + offset:05d}.vtr", x, y, z, rho, p, vel)
+                with open(data / "diagnostics_fp64.d", "a") as f:
+                    d = s.diagnostics()
+                    f.write(f"{nt * step * spec.dt:.16e} {d['ke']:.16e} {d['entropy']:.16e} {d['enstrophy']:.16e} "
+                            f"{d['max_wave_speed']:.16e}\n")
+
+            output(0)
+            for t2 in range(1, np_ + 1):
+                s.step(nt)
+                ok, bad, cnt = s.check_state()
+                if not ok:
+                    raise RuntimeError(f"non-physical state after {t2 * nt} steps: {cnt} cells, first at {bad}")
+                output(t2)
+            io_vtr.write_restart(self.workdir / "recal" / "Q00001.dat", s.get_state())
+
+    def jacobian(self) -> float:
+        s = self.spec
+        dx = s.Lx / (s.nx - s.order)
+        dy = s.Ly / (s.ny - s.order)
+        dz = s.Lz / (s.nz - s.order)
+        return 8.0 / ((dx + dx) * (dy + dy) * 2.0 * dz)
+
+    def __repr__(self) -> str:
+        return f"Case(source={self.source.name!r}, workdir={str(self.workdir)!r}, params={self.params})"

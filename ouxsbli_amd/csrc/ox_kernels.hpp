@@ -8,8 +8,8 @@
 //          (FP64 plain form, FP32 in the FP32-bulk form).
 //   Racc   residual accumulator in the state layout, FP64.
 //
-// One stage = k_prim (QJ -> primitives, whole grid) [+ k_sensor] ->
-// k_sweep_x (LDS row tiles; writes R = c_x*dE) -> k_sweep_line<DIR=1>
+// One stage = k_prim (QJ -> primitives, whole grid) ->
+// k_sweep_x (LDS row tiles; Ducros sensor; writes R = c_x*dE) -> k_sweep_line<DIR=1>
 // (register sliding window along y; R += c_y*dF) -> k_sweep_line<DIR=2>
 // (sliding window along z; R += c_z*dG, then the RK combination) ->
 // k_ghost_fill.  Face fluxes (convective minus viscous) live only in
@@ -79,40 +79,6 @@ __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ QJ, Pri
 }
 
 // ---------------------------------------------------------------------------
-// Ducros sensor.  calc_Ducros (3D_solver/src/calc_hybrid.f90:11-72), over
-// the cells the reference computes (Fortran 2..n-1); the boundary copies are
-// never read by the periodic kernels.
-// ---------------------------------------------------------------------------
-template <class S, class R>
-__global__ __launch_bounds__(256) void k_sensor(Prim<S> P, Geom g) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
-  if (i > g.nx - 2 || j > g.ny - 2) return;
-  const int m = cidx(g, i, j, k);
-  const int sy = g.nx, sz = g.nx * g.ny;
-  const R dxt = R(0.25) * (R(g.inv_dx) + R(g.inv_dx));
-  const R dyt = R(0.25) * (R(g.inv_dy) + R(g.inv_dy));
-  const R dzt = R(0.25) * (R(g.inv_dz) + R(g.inv_dz));
-  const R dudx = (-R(P.u[m - 1]) + R(P.u[m + 1])) * dxt;
-  const R dvdx = (-R(P.v[m - 1]) + R(P.v[m + 1])) * dxt;
-  const R dwdx = (-R(P.w[m - 1]) + R(P.w[m + 1])) * dxt;
-  const R dudy = (-R(P.u[m - sy]) + R(P.u[m + sy])) * dyt;
-  const R dvdy = (-R(P.v[m - sy]) + R(P.v[m + sy])) * dyt;
-  const R dwdy = (-R(P.w[m - sy]) + R(P.w[m + sy])) * dyt;
-  const R dudz = (-R(P.u[m - sz]) + R(P.u[m + sz])) * dzt;
-  const R dvdz = (-R(P.v[m - sz]) + R(P.v[m + sz])) * dzt;
-  const R dwdz = (-R(P.w[m - sz]) + R(P.w[m + sz])) * dzt;
-  const R div = dudx + dvdy + dwdz;
-  const R r1 = dwdy - dvdz;
-  const R r2 = dudz - dwdx;
-  const R r3 = dvdx - dudy;
-  R fd = (div * div) / (div * div + (r1 * r1 + r2 * r2 + r3 * r3) + R(1.0e-12));
-  fd = fmin2(R(1), fd);
-  P.s[m] = (S)fd;
-}
-
-// ---------------------------------------------------------------------------
 // One cell's worth of window data.
 // ---------------------------------------------------------------------------
 template <class R, int DIR, int ORDER, int SCHEME, int VISC, int VO> struct CellData {
@@ -130,8 +96,8 @@ __device__ __forceinline__ R dcross(const PhysC<R>& c, const S* __restrict__ f, 
 
 // Loads the primitive values (and, for viscous runs, the cross derivatives a
 // face in direction DIR needs) of cell (i,j,k) = linear m.
-template <class R, class S, int DIR, int ORDER, int SCHEME, int VISC, int VO>
-__device__ __forceinline__ void load_cell(const Prim<S>& P, const Geom& g, const PhysC<R>& c, int m,
+template <class R, class S, int DIR, int ORDER, int SCHEME, int VISC, int VO, bool LOAD_S = true>
+__device__ __forceinline__ void load_cell(const Prim<S>& P, const Geom& g, const PhysC<R>& c, int m, int ms,
                                           CellData<R, DIR, ORDER, SCHEME, VISC, VO>& d) {
   d.rho = R(P.rho[m]);
   d.u = R(P.u[m]);
@@ -139,7 +105,7 @@ __device__ __forceinline__ void load_cell(const Prim<S>& P, const Geom& g, const
   d.w = R(P.w[m]);
   d.p = R(P.p[m]);
   if constexpr (SCHEME == KEEP || VISC) d.T = R(P.T[m]);
-  if constexpr (SCHEME == SLAU && ORDER > 2) d.s = R(P.s[m]);
+  if constexpr (LOAD_S && SCHEME == SLAU && ORDER > 2) d.s = R(P.s[ms]);
   if constexpr (VISC) {
     d.mu = R(P.mu[m]);
     const int sx = 1, sy = g.nx, sz = g.nx * g.ny;
@@ -215,13 +181,47 @@ __device__ __forceinline__ void face_flux(const PhysC<R>& c, int tvd, int varian
 constexpr int XT = 63;  // cells per x tile
 constexpr int XROWS = 4;
 
+// Ducros sensor of one cell (calc_Ducros, 3D_solver/src/calc_hybrid.f90:11-72)
+// from its six face neighbours' velocities.
+template <class R>
+__device__ __forceinline__ R ducros(const Geom& g, R uxm, R uxp, R vxm, R vxp, R wxm, R wxp, R uym, R uyp, R vym,
+                                    R vyp, R wym, R wyp, R uzm, R uzp, R vzm, R vzp, R wzm, R wzp) {
+  const R dxt = R(0.25) * (R(g.inv_dx) + R(g.inv_dx));
+  const R dyt = R(0.25) * (R(g.inv_dy) + R(g.inv_dy));
+  const R dzt = R(0.25) * (R(g.inv_dz) + R(g.inv_dz));
+  const R dudx = (-uxm + uxp) * dxt;
+  const R dvdx = (-vxm + vxp) * dxt;
+  const R dwdx = (-wxm + wxp) * dxt;
+  const R dudy = (-uym + uyp) * dyt;
+  const R dvdy = (-vym + vyp) * dyt;
+  const R dwdy = (-wym + wyp) * dyt;
+  const R dudz = (-uzm + uzp) * dzt;
+  const R dvdz = (-vzm + vzp) * dzt;
+  const R dwdz = (-wzm + wzp) * dzt;
+  const R div = dudx + dvdy + dwdz;
+  const R r1 = dwdy - dvdz;
+  const R r2 = dudz - dwdx;
+  const R r3 = dvdx - dudy;
+  R fd = (div * div) / (div * div + (r1 * r1 + r2 * r2 + r3 * r3) + R(1.0e-12));
+  return fmin2(R(1), fd);
+}
+
+// x sweep: one wavefront per grid row; 64 faces -> 63 cells per tile; the
+// right-face flux of each cell comes from the neighbouring lane by a
+// cross-lane shuffle.  Writes Racc = c_x * (E_right - E_left) (the first term
+// of calc_R).  For SLAU the Ducros sensor is computed here for the row (its x
+// neighbours are in the LDS row) and stored for the y and z sweeps.
+// The LDS tile is structure-of-arrays so that lane-consecutive reads are
+// bank-conflict free.
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
 __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__ Racc, Geom g, PhysC<R> c, int tvd,
                                                  int variant, double cx) {
   using St = Sten<ORDER, VISC, VO>;
   using CD = CellData<R, 0, ORDER, SCHEME, VISC, VO>;
+  constexpr bool SENS = (SCHEME == SLAU && ORDER > 2);
   constexpr int W = XT + St::SW;
-  __shared__ CD tile[XROWS][W];
+  constexpr int NF = 12;  // rho,u,v,w,p,T,mu,s,d1..d4
+  __shared__ R tile[NF][XROWS][W];
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = g.h + blockIdx.x * XT;
   const int j = g.h + blockIdx.y * XROWS + ty;
@@ -229,18 +229,55 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__
   const bool rowok = j < g.h + g.iy;
   for (int m = tx; m < W; m += 64) {
     const int i = i0 - St::HW + m;
-    if (rowok && i < g.nx) load_cell<R, S, 0, ORDER, SCHEME, VISC, VO>(P, g, c, cidx(g, i, j, k), tile[ty][m]);
+    if (rowok && i < g.nx) {
+      CD d;
+      load_cell<R, S, 0, ORDER, SCHEME, VISC, VO, false>(P, g, c, cidx(g, i, j, k), 0, d);
+      tile[0][ty][m] = d.rho; tile[1][ty][m] = d.u; tile[2][ty][m] = d.v; tile[3][ty][m] = d.w;
+      tile[4][ty][m] = d.p;
+      if constexpr (SCHEME == KEEP || VISC) tile[5][ty][m] = d.T;
+      if constexpr (VISC) {
+        tile[6][ty][m] = d.mu;
+        tile[8][ty][m] = d.d1; tile[9][ty][m] = d.d2; tile[10][ty][m] = d.d3; tile[11][ty][m] = d.d4;
+      }
+    }
   }
   __syncthreads();
+  if constexpr (SENS) {
+    // sensor at tile positions HW-1 .. HW+63 (the cells either side of the 64 faces)
+    for (int m = St::HW - 1 + tx; m <= St::HW + 63; m += 64) {
+      const int i = i0 - St::HW + m;
+      if (rowok && i < g.nx - 1) {
+        const int q = cidx(g, i, j, k);
+        const int sy = g.nx, sz = g.nx * g.ny;
+        const R fd = ducros<R>(g, tile[1][ty][m - 1], tile[1][ty][m + 1], tile[2][ty][m - 1], tile[2][ty][m + 1],
+                               tile[3][ty][m - 1], tile[3][ty][m + 1], R(P.u[q - sy]), R(P.u[q + sy]), R(P.v[q - sy]),
+                               R(P.v[q + sy]), R(P.w[q - sy]), R(P.w[q + sy]), R(P.u[q - sz]), R(P.u[q + sz]),
+                               R(P.v[q - sz]), R(P.v[q + sz]), R(P.w[q - sz]), R(P.w[q + sz]));
+        tile[7][ty][m] = fd;
+        if (i >= i0 && i < i0 + XT && i < g.h + g.ix) P.s[q] = (S)fd;  // each cell stored by one tile
+      }
+    }
+    __syncthreads();
+  }
   if (!rowok) return;
   // face f = i0 - 1 + tx: window cells f-HW+1 .. f+HW = tile positions tx .. tx+SW-1
   R F[5] = {R(0), R(0), R(0), R(0), R(0)};
   const int f = i0 - 1 + tx;
   const bool face_ok = f <= g.h + g.ix - 1;
-  CD win[St::SW];
   if (face_ok) {
+    CD win[St::SW];
 #pragma unroll
-    for (int m = 0; m < St::SW; ++m) win[m] = tile[ty][tx + m];
+    for (int m = 0; m < St::SW; ++m) {
+      win[m].rho = tile[0][ty][tx + m]; win[m].u = tile[1][ty][tx + m]; win[m].v = tile[2][ty][tx + m];
+      win[m].w = tile[3][ty][tx + m]; win[m].p = tile[4][ty][tx + m];
+      if constexpr (SCHEME == KEEP || VISC) win[m].T = tile[5][ty][tx + m];
+      if constexpr (SENS) win[m].s = tile[7][ty][tx + m];
+      if constexpr (VISC) {
+        win[m].mu = tile[6][ty][tx + m];
+        win[m].d1 = tile[8][ty][tx + m]; win[m].d2 = tile[9][ty][tx + m];
+        win[m].d3 = tile[10][ty][tx + m]; win[m].d4 = tile[11][ty][tx + m];
+      }
+    }
     face_flux<R, 0, ORDER, SCHEME, VISC, VO>(c, tvd, variant, R(g.inv_dx), win, F);
   }
   double Fr[5];
@@ -262,7 +299,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__
 // ---------------------------------------------------------------------------
 constexpr int LSEG = 32;
 
-template <class R, class S, int ORDER, int SCHEME, int VISC, int VO, int DIR>
+template <class R, class S, int ORDER, int SCHEME, int VISC, int VO, int DIR, int EP>
 __global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, double* __restrict__ Racc, Geom g, PhysC<R> c,
                                                     int tvd, int variant, double cd, StageArgs sa) {
   using St = Sten<ORDER, VISC, VO>;
@@ -279,13 +316,20 @@ __global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, double* __restric
   const int qbase = DIR == 1 ? qidx(g, i, 0, 0, t) : qidx(g, i, 0, t, 0);
   const int qstride = DIR == 1 ? 5 * g.nx : 5 * g.nx * g.ny;
   const R inv_d = DIR == 1 ? R(g.inv_dy) : R(g.inv_dz);
+  // the sensor is stored for interior cells only; a ghost cell's sensor is
+  // bitwise that of its periodic image (its inputs are exact copies)
+  auto wrap = [&](int sp) { return sp < g.h ? sp + nd : (sp >= g.h + nd ? sp - nd : sp); };
   CD win[St::SW];
 #pragma unroll
-  for (int m = 0; m < St::SW - 1; ++m)
-    load_cell<R, S, DIR, ORDER, SCHEME, VISC, VO>(P, g, c, mbase + (s0 - St::HW + m) * stride, win[m]);
+  for (int m = 0; m < St::SW - 1; ++m) {
+    const int sp = s0 - St::HW + m;
+    load_cell<R, S, DIR, ORDER, SCHEME, VISC, VO>(P, g, c, mbase + sp * stride, mbase + wrap(sp) * stride, win[m]);
+  }
   R Fp[5];
   for (int f = s0 - 1; f < s_end; ++f) {
-    load_cell<R, S, DIR, ORDER, SCHEME, VISC, VO>(P, g, c, mbase + (f + St::HW) * stride, win[St::SW - 1]);
+    const int sp = f + St::HW;
+    load_cell<R, S, DIR, ORDER, SCHEME, VISC, VO>(P, g, c, mbase + sp * stride, mbase + wrap(sp) * stride,
+                                                  win[St::SW - 1]);
     R F[5];
     face_flux<R, DIR, ORDER, SCHEME, VISC, VO>(c, tvd, variant, inv_d, win, F);
     if (f >= s0) {
@@ -298,23 +342,23 @@ __global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, double* __restric
         for (int l = 0; l < 5; ++l) Racc[q + l * g.nx] = Rl[l];
       } else {
         // stage epilogue (3D_solver/src/calc_steps.f90:60-69, :98-106, :137-145, :173-183)
-        if (sa.ep == EP_RK3_S1) {
+        if constexpr (EP == EP_RK3_S1) {
 #pragma unroll
           for (int l = 0; l < 5; ++l) sa.Qout[q + l * g.nx] = sa.Qin[q + l * g.nx] - Rl[l];
-        } else if (sa.ep == EP_RK3_S23) {
+        } else if constexpr (EP == EP_RK3_S23) {
 #pragma unroll
           for (int l = 0; l < 5; ++l) {
             const int a = q + l * g.nx;
             sa.Qout[a] = (sa.c1 * sa.Qin[a] + sa.c2 * sa.Qout[a] - Rl[l]) * sa.c4inv;
           }
-        } else if (sa.ep == EP_RK4_S) {
+        } else if constexpr (EP == EP_RK4_S) {
 #pragma unroll
           for (int l = 0; l < 5; ++l) {
             const int a = q + l * g.nx;
             sa.Qout[a] = sa.Qin[a] - sa.c1 * Rl[l];
             sa.Rs[a] = sa.Rs[a] + sa.c2 * Rl[l];
           }
-        } else if (sa.ep == EP_RK4_S4) {
+        } else if constexpr (EP == EP_RK4_S4) {
 #pragma unroll
           for (int l = 0; l < 5; ++l) {
             const int a = q + l * g.nx;

@@ -31,20 +31,25 @@ void run_stage(Core& C, const double* Qsrc, const StageArgs& sa, double cx, doub
   const dim3 blk(64, 4, 1);
   k_prim<S, (VISC != 0)><<<dim3(cdiv(g.nx, 64), cdiv(g.ny, 4), g.nz), blk, 0, C.stream>>>(Qsrc, P, g, c64);
   prof_mark(C, "prim");
-  if constexpr (SCHEME == SLAU && ORDER > 2) {
-    k_sensor<S, R><<<dim3(cdiv(g.nx - 2, 64), cdiv(g.ny - 2, 4), g.nz - 2), blk, 0, C.stream>>>(P, g);
-    prof_mark(C, "sensor");
-  }
   k_sweep_x<R, S, ORDER, SCHEME, VISC, VO>
       <<<dim3(cdiv(g.ix, XT), cdiv(g.iy, XROWS), g.iz), blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd, C.variant, cx);
   prof_mark(C, "sweep_x");
-  k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 1>
+  k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 1, EP_RHS>
       <<<dim3(cdiv(g.ix, 64), cdiv(g.iz, 4), cdiv(g.iy, LSEG)), blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd,
                                                                                     C.variant, cy, sa);
   prof_mark(C, "sweep_y");
-  k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 2>
-      <<<dim3(cdiv(g.ix, 64), cdiv(g.iy, 4), cdiv(g.iz, LSEG)), blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd,
-                                                                                    C.variant, cz, sa);
+  const dim3 gz(cdiv(g.ix, 64), cdiv(g.iy, 4), cdiv(g.iz, LSEG));
+#define OX_Z(E)                                                                                                 \
+  k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 2, E><<<gz, blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd, C.variant, \
+                                                                              cz, sa)
+  switch (sa.ep) {
+    case EP_RK3_S1: OX_Z(EP_RK3_S1); break;
+    case EP_RK3_S23: OX_Z(EP_RK3_S23); break;
+    case EP_RK4_S: OX_Z(EP_RK4_S); break;
+    case EP_RK4_S4: OX_Z(EP_RK4_S4); break;
+    default: OX_Z(EP_RHS); break;
+  }
+#undef OX_Z
   prof_mark(C, "sweep_z_rk");
   if (sa.ep != EP_RHS) {
     launch_ghost_fill(C, sa.Qout);

@@ -25,7 +25,7 @@ bool Oracle::init(const ox_config& cf) {
   if (cf.visc != OX_VISC_EULER && vorder != 4 && vorder != 6) { err = "visc_order must be 4 or 6"; return false; }
   if (cf.visc != OX_VISC_EULER && vorder == 6 && order < 6) { err = "visc_order 6 needs order 6"; return false; }
   if (cf.visc == OX_VISC_LES) { err = "LES is not in the oracle"; return false; }
-  if (cf.scheme != OX_SCHEME_KEEP && cf.scheme != OX_SCHEME_SLAU) { err = "scheme not in the oracle"; return false; }
+  if (cf.scheme < OX_SCHEME_KEEP || cf.scheme > OX_SCHEME_HYBRID) { err = "unknown scheme"; return false; }
   if (cf.visc_stencil != OX_STENCIL_CENTRAL) { err = "ME4Base not in the oracle"; return false; }
   if (nx < 2 * order + 2 || ny < 2 * order + 2 || nz < 2 * order + 2) { err = "grid too small"; return false; }
   c.init(cf);
@@ -156,36 +156,58 @@ void Oracle::ducros() {
 // ---------------------------------------------------------------------------
 static const double kNormal[3][5] = {{0, 1, 0, 0, 0}, {0, 0, 1, 0, 0}, {0, 0, 0, 1, 0}};
 
+// MUSCL left/right states of the five primitives: interp2/4/6 of the SLAU
+// kernel (calc_slau_kernel_internal.f90.fypp:31-81) and delta_r2/4/6 of the
+// Roe and hybrid kernels (calc_roe_kernel_internal.f90.fypp:28-55) -- the same
+// reconstruction; only SLAU replaces the sensor by the wiggle flag afterwards.
+static void reconstruct(const Oracle& o, int n, double st[6][6], double fsens, double* L, double* Rr) {
+  if (n == 2) {
+    for (int v = 0; v < 5; ++v) { L[v] = st[v][0]; Rr[v] = st[v][1]; }
+  } else if (n == 4) {
+    for (int v = 0; v < 5; ++v) delta4(o.c, o.cfg.tvd, fsens, st[v], L[v], Rr[v]);
+  } else {
+    for (int v = 0; v < 5; ++v) delta6(o.c, o.cfg.tvd, fsens, st[v], L[v], Rr[v]);
+  }
+}
+
 static void conv_face(const Oracle& o, int dir, double st[6][6], double fsens, double* out) {
   const Consts& c = o.c;
   const double* N = kNormal[dir];
   const int n = 2 * o.io + 2;
   const double* uu = st[1 + dir];
-  if (o.cfg.scheme == OX_SCHEME_KEEP) {
+  int scheme = o.cfg.scheme;
+  // calc_hybrid_{x,y,z}_in (calc_hybrid_kernel_internal.f90.fypp:96-121): KEEP where
+  // the face sensor is <= threshold, else MUSCL + SLAU with HR = 1
+  if (scheme == OX_SCHEME_HYBRID) scheme = (fsens <= c.threshold) ? OX_SCHEME_KEEP : -1;
+  if (scheme == OX_SCHEME_KEEP) {
     if (n == 2) keep2(c, st[0], st[1], st[2], st[3], uu, st[4], st[5], N, out);
     else if (n == 4) keep4(c, st[0], st[1], st[2], st[3], uu, st[4], st[5], N, out);
     else keep6(c, st[0], st[1], st[2], st[3], uu, st[4], st[5], N, out);
     return;
   }
-  // SLAU: interp2/4/6 (:31-81) then SLAU1 / HRSLAU2
   double L[5], Rr[5];
-  double fd = fsens;
-  if (n == 2) {
-    for (int v = 0; v < 5; ++v) { L[v] = st[v][0]; Rr[v] = st[v][1]; }
-    fd = 1.0;
-  } else if (n == 4) {
-    for (int v = 0; v < 5; ++v) delta4(c, o.cfg.tvd, fd, st[v], L[v], Rr[v]);
-    fd = wiggle_detector(&st[4][0]);
-  } else {
-    for (int v = 0; v < 5; ++v) delta6(c, o.cfg.tvd, fd, st[v], L[v], Rr[v]);
-    fd = wiggle_detector(&st[4][1]);
+  reconstruct(o, n, st, fsens, L, Rr);
+  if (scheme == OX_SCHEME_ROE) {
+    // calc_roe_{x,y,z}_in: local frame (normal, t1, t2) in cyclic order
+    const int a = 1 + dir, b = 1 + (dir + 1) % 3, d = 1 + (dir + 2) % 3;
+    double loc[5];
+    roe_flux(c, L[0], Rr[0], L[a], Rr[a], L[b], Rr[b], L[d], Rr[d], L[4], Rr[4], loc);
+    out[0] = loc[0]; out[a] = loc[1]; out[b] = loc[2]; out[d] = loc[3]; out[4] = loc[4];
+    return;
   }
-  slau_flux(c, o.cfg.slau_variant, L[0], Rr[0], L[1], Rr[1], L[2], Rr[2], L[3], Rr[3], L[1 + dir], Rr[1 + dir], L[4],
-            Rr[4], N, fd, out);
+  double HR;
+  int variant = o.cfg.slau_variant;
+  if (scheme == OX_SCHEME_SLAU) {
+    HR = n == 2 ? 1.0 : (n == 4 ? wiggle_detector(&st[4][0]) : wiggle_detector(&st[4][1]));
+  } else {  // hybrid, shock branch
+    HR = 1.0;
+  }
+  slau_flux(c, variant, L[0], Rr[0], L[1], Rr[1], L[2], Rr[2], L[3], Rr[3], L[1 + dir], Rr[1 + dir], L[4], Rr[4], N,
+            HR, out);
 }
 
 void Oracle::conv_x() {
-  const bool slau = cfg.scheme == OX_SCHEME_SLAU;
+  const bool slau = cfg.scheme != OX_SCHEME_KEEP;
 #pragma omp parallel for schedule(static)
   for (int k = 2; k <= nz - 1; ++k)
     for (int j = 2; j <= ny - 1; ++j)
@@ -204,7 +226,7 @@ void Oracle::conv_x() {
 }
 
 void Oracle::conv_y() {
-  const bool slau = cfg.scheme == OX_SCHEME_SLAU;
+  const bool slau = cfg.scheme != OX_SCHEME_KEEP;
 #pragma omp parallel for schedule(static)
   for (int k = 2; k <= nz - 1; ++k)
     for (int j = io + 1; j <= ny - (io + 1); ++j)
@@ -223,7 +245,7 @@ void Oracle::conv_y() {
 }
 
 void Oracle::conv_z() {
-  const bool slau = cfg.scheme == OX_SCHEME_SLAU;
+  const bool slau = cfg.scheme != OX_SCHEME_KEEP;
 #pragma omp parallel for schedule(static)
   for (int k = io + 1; k <= nz - (io + 1); ++k)
     for (int j = 2; j <= ny - 1; ++j)
@@ -714,6 +736,17 @@ void oxo_slau_flux(const ox_config* cfg, int order, const double* st /*[6][6]*/,
   o.c.init(*cfg);
   o.cfg = *cfg;
   o.cfg.scheme = OX_SCHEME_SLAU;
+  o.io = order / 2 - 1;
+  double s[6][6];
+  std::memcpy(s, st, sizeof(s));
+  oxo::conv_face(o, dir, s, fsens, out);
+}
+
+void oxo_conv_flux(const ox_config* cfg, int order, const double* st /*[6][6]*/, int dir, double fsens,
+                   double* out) {
+  oxo::Oracle o;
+  o.c.init(*cfg);
+  o.cfg = *cfg;
   o.io = order / 2 - 1;
   double s[6][6];
   std::memcpy(s, st, sizeof(s));

@@ -329,6 +329,66 @@ inline void slau_flux(const Consts& c, int variant, double rho1, double rho2, do
 }
 
 // ---------------------------------------------------------------------------
+// Roe flux.  3D_solver/src/calc_roe_3d.f90:1-88.  Arguments in the face's
+// local frame: un = normal velocity, (t1, t2) = the other two components in
+// cyclic order; o[5] = (mass, normal, t1, t2, energy) flux components.
+// ---------------------------------------------------------------------------
+inline void roe_flux(const Consts& c, double rho1, double rho2, double u1, double u2, double v1, double v2,
+                     double w1, double w2, double p1, double p2, double* F) {
+  const double k1 = 0.5 * (u1 * u1 + v1 * v1 + w1 * w1);
+  const double k2 = 0.5 * (u2 * u2 + v2 * v2 + w2 * w2);
+  double dQ[5];
+  // calc_Roe_common (:57-83)
+  const double e1 = p1 * c.over_gamma_1 + rho1 * k1;
+  const double e2 = p2 * c.over_gamma_1 + rho2 * k2;
+  {
+    const double rhou1 = rho1 * u1;
+    const double rhou2 = rho2 * u2;
+    F[0] = 0.5 * (rhou1 + rhou2);
+    F[1] = 0.5 * (rhou1 * u1 + rhou2 * u2 + (p1 + p2));
+    F[2] = 0.5 * (rhou1 * v1 + rhou2 * v2);
+    F[3] = 0.5 * (rhou1 * w1 + rhou2 * w2);
+    F[4] = 0.5 * ((e1 + p1) * u1 + (e2 + p2) * u2);
+  }
+  dQ[0] = -rho1 + rho2;
+  dQ[1] = -(rho1 * u1) + rho2 * u2;
+  dQ[2] = -(rho1 * v1) + rho2 * v2;
+  dQ[3] = -(rho1 * w1) + rho2 * w2;
+  dQ[4] = -e1 + e2;
+  // calc_Roe_ave (:39-54)
+  const double H1 = c.gamma_over_gamma_1 * p1 / rho1 + k1;
+  const double H2 = c.gamma_over_gamma_1 * p2 / rho2 + k2;
+  const double rhol = fsqrt(rho1);
+  const double rhor = fsqrt(rho2);
+  const double rr = 1.0 / (rhol + rhor);
+  const double u = (rhol * u1 + rhor * u2) * rr;
+  const double v = (rhol * v1 + rhor * v2) * rr;
+  const double w = (rhol * w1 + rhor * w2) * rr;
+  const double H = (rhol * H1 + rhor * H2) * rr;
+  const double cr = fsqrt(c.gamma_1 * (H - 0.5 * (u * u + v * v + w * w)));
+  // calc_RARinv (:1-36)
+  const double q2 = 0.5 * (u * u + v * v + w * w);
+  const double oc = 1.0 / cr;
+  const double b2 = c.gamma_1 * oc * oc;
+  const double b1 = q2 * b2;
+  const double R1 = 0.5 * ((b1 + u * oc) * dQ[0] - (oc + b2 * u) * dQ[1] - b2 * v * dQ[2] - b2 * w * dQ[3] + b2 * dQ[4]);
+  const double R2 = oc * (w * dQ[0] - dQ[3]);
+  const double R3 = oc * (-(v * dQ[0]) + dQ[2]);
+  const double R4 = (1.0 - b1) * dQ[0] + b2 * (u * dQ[1] + v * dQ[2] + w * dQ[3] - dQ[4]);
+  const double R5 = 0.5 * ((b1 - u * oc) * dQ[0] + (oc - b2 * u) * dQ[1] - b2 * (v * dQ[2] + w * dQ[3] - dQ[4]));
+  const double a1 = fabs_(u - cr);
+  const double a2 = fabs_(u);
+  const double a3 = fabs_(u + cr);
+  const double a2c = a2 * cr;
+  const double cu = cr * u;
+  F[0] = F[0] - 0.5 * (a1 * R1 + a2 * R4 + a3 * R5);
+  F[1] = F[1] - 0.5 * (a1 * (u - cr) * R1 + a2 * u * R4 + a3 * (u + cr) * R5);
+  F[2] = F[2] - 0.5 * (a1 * v * R1 + a2c * R3 + a2 * v * R4 + a3 * v * R5);
+  F[3] = F[3] - 0.5 * (a1 * w * R1 - a2c * R2 + a2 * w * R4 + a3 * w * R5);
+  F[4] = F[4] - 0.5 * (a1 * (H - cu) * R1 + a2 * (-(w * cr * R2) + v * cr * R3 + q2 * R4) + a3 * (H + cu) * R5);
+}
+
+// ---------------------------------------------------------------------------
 // Central viscous helpers.  3D_solver/src/calc_visc_cent.f90.fypp:2-91.
 // ---------------------------------------------------------------------------
 inline double vinterp(int N, const double* a) {

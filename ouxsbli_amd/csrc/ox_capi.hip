@@ -133,8 +133,7 @@ int validate(const ox_config& c) {
   if (c.visc == OX_VISC_LES) return fail(OX_EUNSUPPORTED, "LES is not built yet");
   if (c.visc_stencil != OX_STENCIL_CENTRAL && c.visc != OX_VISC_EULER)
     return fail(OX_EUNSUPPORTED, "ME4Base viscous stencil is not built yet");
-  if (c.scheme != OX_SCHEME_KEEP && c.scheme != OX_SCHEME_SLAU)
-    return fail(OX_EUNSUPPORTED, "scheme not built (KEEP and SLAU are)");
+  if (c.scheme < OX_SCHEME_KEEP || c.scheme > OX_SCHEME_HYBRID) return fail(OX_EINVAL, "unknown scheme");
   if (c.tvd < 0 || c.tvd > 2) return fail(OX_EINVAL, "tvd must be 0, 1 or 2");
   if (c.slau_variant < 0 || c.slau_variant > 1) return fail(OX_EINVAL, "slau_variant must be 0 or 1");
   if (c.precision != OX_PREC_FP64 && c.precision != OX_PREC_FP32_BULK) return fail(OX_EINVAL, "bad precision");

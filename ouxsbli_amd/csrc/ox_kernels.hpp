@@ -30,6 +30,12 @@ template <class S> struct Prim {
 
 constexpr int KEEP = OX_SCHEME_KEEP;
 constexpr int SLAU = OX_SCHEME_SLAU;
+constexpr int ROE = OX_SCHEME_ROE;
+constexpr int HYBRID = OX_SCHEME_HYBRID;
+// schemes that read the Ducros sensor (calc_flux_base.f90.fypp:56-58)
+template <int SCHEME, int ORDER> constexpr bool kUsesSensor = SCHEME != KEEP && ORDER > 2;
+// schemes whose face flux reads T (KEEP's internal-energy term)
+template <int SCHEME> constexpr bool kUsesT = SCHEME == KEEP || SCHEME == HYBRID;
 
 // Stencil width of a face: the convective stencil (ORDER cells) and the
 // viscous one (VO cells) are both centred on the face.
@@ -104,8 +110,8 @@ __device__ __forceinline__ void load_cell(const Prim<S>& P, const Geom& g, const
   d.v = R(P.v[m]);
   d.w = R(P.w[m]);
   d.p = R(P.p[m]);
-  if constexpr (SCHEME == KEEP || VISC) d.T = R(P.T[m]);
-  if constexpr (LOAD_S && SCHEME == SLAU && ORDER > 2) d.s = R(P.s[ms]);
+  if constexpr (kUsesT<SCHEME> || VISC) d.T = R(P.T[m]);
+  if constexpr (LOAD_S && kUsesSensor<SCHEME, ORDER>) d.s = R(P.s[ms]);
   if constexpr (VISC) {
     d.mu = R(P.mu[m]);
     const int sx = 1, sy = g.nx, sz = g.nx * g.ny;
@@ -143,16 +149,25 @@ __device__ __forceinline__ void face_flux(const PhysC<R>& c, int tvd, int varian
     v[m] = w[St::CO + m].v;
     wv[m] = w[St::CO + m].w;
     p[m] = w[St::CO + m].p;
-    if constexpr (SCHEME == KEEP) T[m] = w[St::CO + m].T;
+    if constexpr (kUsesT<SCHEME>) T[m] = w[St::CO + m].T;
   }
+  R fs = R(0);
+  if constexpr (kUsesSensor<SCHEME, ORDER>) fs = R(0.5) * (w[St::HW - 1].s + w[St::HW].s);
   if constexpr (SCHEME == KEEP) {
-    if constexpr (N == 2) keep2<R, DIR>(c, rho, u, v, wv, p, T, F);
-    else if constexpr (N == 4) keep4<R, DIR>(c, rho, u, v, wv, p, T, F);
-    else keep6<R, DIR>(c, rho, u, v, wv, p, T, F);
-  } else {
-    R fs = R(0);
-    if constexpr (N > 2) fs = R(0.5) * (w[St::HW - 1].s + w[St::HW].s);
+    keep_face<R, DIR, N>(c, rho, u, v, wv, p, T, F);
+  } else if constexpr (SCHEME == SLAU) {
     slau_face<R, DIR, N>(c, tvd, variant, rho, u, v, wv, p, fs, F);
+  } else if constexpr (SCHEME == ROE) {
+    roe_face<R, DIR, N>(c, tvd, rho, u, v, wv, p, fs, F);
+  } else {
+    // calc_hybrid_{x,y,z}_in (calc_hybrid_kernel_internal.f90.fypp:96-121)
+    if (fs <= c.threshold) {
+      keep_face<R, DIR, N>(c, rho, u, v, wv, p, T, F);
+    } else {
+      R L[5], Rr[5];
+      reconstruct<R, N>(c, tvd, fs, rho, u, v, wv, p, L, Rr);
+      slau<R, DIR>(c, variant, L[0], Rr[0], L[1], Rr[1], L[2], Rr[2], L[3], Rr[3], L[4], Rr[4], R(1), F);
+    }
   }
   if constexpr (VISC) {
     constexpr int M = VO;
@@ -218,7 +233,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__
                                                  int variant, double cx) {
   using St = Sten<ORDER, VISC, VO>;
   using CD = CellData<R, 0, ORDER, SCHEME, VISC, VO>;
-  constexpr bool SENS = (SCHEME == SLAU && ORDER > 2);
+  constexpr bool SENS = kUsesSensor<SCHEME, ORDER>;
   constexpr int W = XT + St::SW;
   constexpr int NF = 12;  // rho,u,v,w,p,T,mu,s,d1..d4
   __shared__ R tile[NF][XROWS][W];
@@ -234,7 +249,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__
       load_cell<R, S, 0, ORDER, SCHEME, VISC, VO, false>(P, g, c, cidx(g, i, j, k), 0, d);
       tile[0][ty][m] = d.rho; tile[1][ty][m] = d.u; tile[2][ty][m] = d.v; tile[3][ty][m] = d.w;
       tile[4][ty][m] = d.p;
-      if constexpr (SCHEME == KEEP || VISC) tile[5][ty][m] = d.T;
+      if constexpr (kUsesT<SCHEME> || VISC) tile[5][ty][m] = d.T;
       if constexpr (VISC) {
         tile[6][ty][m] = d.mu;
         tile[8][ty][m] = d.d1; tile[9][ty][m] = d.d2; tile[10][ty][m] = d.d3; tile[11][ty][m] = d.d4;
@@ -270,7 +285,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__
     for (int m = 0; m < St::SW; ++m) {
       win[m].rho = tile[0][ty][tx + m]; win[m].u = tile[1][ty][tx + m]; win[m].v = tile[2][ty][tx + m];
       win[m].w = tile[3][ty][tx + m]; win[m].p = tile[4][ty][tx + m];
-      if constexpr (SCHEME == KEEP || VISC) win[m].T = tile[5][ty][tx + m];
+      if constexpr (kUsesT<SCHEME> || VISC) win[m].T = tile[5][ty][tx + m];
       if constexpr (SENS) win[m].s = tile[7][ty][tx + m];
       if constexpr (VISC) {
         win[m].mu = tile[6][ty][tx + m];

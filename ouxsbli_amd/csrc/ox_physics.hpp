@@ -34,7 +34,7 @@ OX_HD float fsqrt(float x) { return __builtin_sqrtf(x); }
 
 // Per-case constants in the arithmetic type R (src/mod_constant.f90.fypp:18-35).
 template <class R> struct PhysC {
-  R gamma, gamma_1, R_over_gamma_1, Rgas, Cp_over_Pr, mu0c;
+  R gamma, gamma_1, over_gamma_1, R_over_gamma_1, Rgas, Cp_over_Pr, mu0c;
   R one_third, two_third, one_sixth, one_twelfth, one_sixty;
   R one_24, one_48, one_60, one_120, one_240, seven_twelfth;
   R gamma_over_gamma_1, minmod_b, threshold;
@@ -42,7 +42,7 @@ template <class R> struct PhysC {
 
 template <class R> __host__ PhysC<R> make_physc(const HostConsts& h) {
   PhysC<R> c;
-  c.gamma = (R)h.gamma; c.gamma_1 = (R)h.gamma_1; c.R_over_gamma_1 = (R)h.R_over_gamma_1; c.Rgas = (R)h.Rgas;
+  c.gamma = (R)h.gamma; c.gamma_1 = (R)h.gamma_1; c.over_gamma_1 = (R)h.over_gamma_1; c.R_over_gamma_1 = (R)h.R_over_gamma_1; c.Rgas = (R)h.Rgas;
   c.Cp_over_Pr = (R)h.Cp_over_Pr; c.mu0c = (R)h.mu0c;
   c.one_third = (R)h.one_third; c.two_third = (R)h.two_third; c.one_sixth = (R)h.one_sixth;
   c.one_twelfth = (R)h.one_twelfth; c.one_sixty = (R)h.one_sixty;
@@ -134,6 +134,13 @@ OX_HD void keep6(const PhysC<R>& c, const R* rho, const R* u, const R* v, const 
   F[4] = ene + (R(0.75) * (uu[2] * p[3] + uu[3] * p[2]) -
                 R(0.15) * (uu[2] * p[4] + uu[4] * p[2] + uu[1] * p[3] + uu[3] * p[1]) +
                 (uu[2] * p[5] + uu[5] * p[2] + uu[1] * p[4] + uu[4] * p[1] + uu[0] * p[3] + uu[3] * p[0]) * c.one_60);
+}
+
+template <class R, int DIR, int N>
+OX_HD void keep_face(const PhysC<R>& c, const R* rho, const R* u, const R* v, const R* w, const R* p, const R* T, R* F) {
+  if constexpr (N == 2) keep2<R, DIR>(c, rho, u, v, w, p, T, F);
+  else if constexpr (N == 4) keep4<R, DIR>(c, rho, u, v, w, p, T, F);
+  else keep6<R, DIR>(c, rho, u, v, w, p, T, F);
 }
 
 // ---------------------------------------------------------------------------
@@ -258,6 +265,98 @@ OX_HD void slau(const PhysC<R>& c, int variant, R rho1, R rho2, R u1, R u2, R v1
   const R phi1 = (p1 * c.gamma_over_gamma_1 + rho1 * k1) * over_rho1;
   const R phi2 = (p2 * c.gamma_over_gamma_1 + rho2 * k2) * over_rho2;
   F[4] = mass1 * phi1 + mass2 * phi2;
+}
+
+// ---------------------------------------------------------------------------
+// Roe.  3D_solver/src/calc_roe_3d.f90:1-88, in the face's local frame
+// (u = normal velocity); o[5] = (mass, normal, t1, t2, energy).
+// ---------------------------------------------------------------------------
+template <class R>
+OX_HD void roe_local(const PhysC<R>& c, R rho1, R rho2, R u1, R u2, R v1, R v2, R w1, R w2, R p1, R p2, R* F) {
+  const R k1 = R(0.5) * (u1 * u1 + v1 * v1 + w1 * w1);
+  const R k2 = R(0.5) * (u2 * u2 + v2 * v2 + w2 * w2);
+  const R e1 = p1 * c.over_gamma_1 + rho1 * k1;
+  const R e2 = p2 * c.over_gamma_1 + rho2 * k2;
+  const R rhou1 = rho1 * u1;
+  const R rhou2 = rho2 * u2;
+  F[0] = R(0.5) * (rhou1 + rhou2);
+  F[1] = R(0.5) * (rhou1 * u1 + rhou2 * u2 + (p1 + p2));
+  F[2] = R(0.5) * (rhou1 * v1 + rhou2 * v2);
+  F[3] = R(0.5) * (rhou1 * w1 + rhou2 * w2);
+  F[4] = R(0.5) * ((e1 + p1) * u1 + (e2 + p2) * u2);
+  const R dQ0 = -rho1 + rho2;
+  const R dQ1 = -(rho1 * u1) + rho2 * u2;
+  const R dQ2 = -(rho1 * v1) + rho2 * v2;
+  const R dQ3 = -(rho1 * w1) + rho2 * w2;
+  const R dQ4 = -e1 + e2;
+  const R H1 = c.gamma_over_gamma_1 * p1 / rho1 + k1;
+  const R H2 = c.gamma_over_gamma_1 * p2 / rho2 + k2;
+  const R rhol = fsqrt(rho1);
+  const R rhor = fsqrt(rho2);
+  const R rr = R(1) / (rhol + rhor);
+  const R u = (rhol * u1 + rhor * u2) * rr;
+  const R v = (rhol * v1 + rhor * v2) * rr;
+  const R w = (rhol * w1 + rhor * w2) * rr;
+  const R H = (rhol * H1 + rhor * H2) * rr;
+  const R cr = fsqrt(c.gamma_1 * (H - R(0.5) * (u * u + v * v + w * w)));
+  const R q2 = R(0.5) * (u * u + v * v + w * w);
+  const R oc = R(1) / cr;
+  const R b2 = c.gamma_1 * oc * oc;
+  const R b1 = q2 * b2;
+  const R R1 = R(0.5) * ((b1 + u * oc) * dQ0 - (oc + b2 * u) * dQ1 - b2 * v * dQ2 - b2 * w * dQ3 + b2 * dQ4);
+  const R R2 = oc * (w * dQ0 - dQ3);
+  const R R3 = oc * (-(v * dQ0) + dQ2);
+  const R R4 = (R(1) - b1) * dQ0 + b2 * (u * dQ1 + v * dQ2 + w * dQ3 - dQ4);
+  const R R5 = R(0.5) * ((b1 - u * oc) * dQ0 + (oc - b2 * u) * dQ1 - b2 * (v * dQ2 + w * dQ3 - dQ4));
+  const R a1 = fabs_(u - cr);
+  const R a2 = fabs_(u);
+  const R a3 = fabs_(u + cr);
+  const R a2c = a2 * cr;
+  const R cu = cr * u;
+  F[0] = F[0] - R(0.5) * (a1 * R1 + a2 * R4 + a3 * R5);
+  F[1] = F[1] - R(0.5) * (a1 * (u - cr) * R1 + a2 * u * R4 + a3 * (u + cr) * R5);
+  F[2] = F[2] - R(0.5) * (a1 * v * R1 + a2c * R3 + a2 * v * R4 + a3 * v * R5);
+  F[3] = F[3] - R(0.5) * (a1 * w * R1 - a2c * R2 + a2 * w * R4 + a3 * w * R5);
+  F[4] = F[4] - R(0.5) * (a1 * (H - cu) * R1 + a2 * (-(w * cr * R2) + v * cr * R3 + q2 * R4) + a3 * (H + cu) * R5);
+}
+
+// MUSCL left/right states of the five primitives (delta_r2/4/6 of
+// calc_roe_kernel_internal.f90.fypp:28-55 = the delta4/delta6 of SLAU).
+template <class R, int N>
+OX_HD void reconstruct(const PhysC<R>& c, int tvd, R fsens, const R* rho, const R* u, const R* v, const R* w,
+                       const R* p, R* L, R* Rr) {
+  if constexpr (N == 2) {
+    L[0] = rho[0]; Rr[0] = rho[1]; L[1] = u[0]; Rr[1] = u[1]; L[2] = v[0]; Rr[2] = v[1];
+    L[3] = w[0]; Rr[3] = w[1]; L[4] = p[0]; Rr[4] = p[1];
+  } else if constexpr (N == 4) {
+    delta4(c, tvd, fsens, rho, L[0], Rr[0]);
+    delta4(c, tvd, fsens, u, L[1], Rr[1]);
+    delta4(c, tvd, fsens, v, L[2], Rr[2]);
+    delta4(c, tvd, fsens, w, L[3], Rr[3]);
+    delta4(c, tvd, fsens, p, L[4], Rr[4]);
+  } else {
+    delta6(c, tvd, fsens, rho, L[0], Rr[0]);
+    delta6(c, tvd, fsens, u, L[1], Rr[1]);
+    delta6(c, tvd, fsens, v, L[2], Rr[2]);
+    delta6(c, tvd, fsens, w, L[3], Rr[3]);
+    delta6(c, tvd, fsens, p, L[4], Rr[4]);
+  }
+}
+
+// calc_roe_{x,y,z}_in: MUSCL, then Roe in the local frame (normal, t1, t2)
+// with (t1, t2) cyclic after the normal; outputs mapped back.
+template <class R, int DIR, int N>
+OX_HD void roe_face(const PhysC<R>& c, int tvd, const R* rho, const R* u, const R* v, const R* w, const R* p, R fsens,
+                    R* F) {
+  R L[5], Rr[5], o[5];
+  reconstruct<R, N>(c, tvd, fsens, rho, u, v, w, p, L, Rr);
+  constexpr int a = 1 + DIR, b = 1 + (DIR + 1) % 3, d = 1 + (DIR + 2) % 3;
+  roe_local<R>(c, L[0], Rr[0], L[a], Rr[a], L[b], Rr[b], L[d], Rr[d], L[4], Rr[4], o);
+  F[0] = o[0];
+  F[a] = o[1];
+  F[b] = o[2];
+  F[d] = o[3];
+  F[4] = o[4];
 }
 
 // SLAU face: interp2/4/6 (calc_slau_kernel_internal.f90.fypp:31-81) then SLAU.

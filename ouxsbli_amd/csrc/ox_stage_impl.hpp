@@ -73,6 +73,17 @@ void run_stage(Core& C, const double* Qsrc, const StageArgs& sa, double cx, doub
     case 4114: return &run_stage<R, S, 4, SLAU, 1, 4>;                        \
     case 6114: return &run_stage<R, S, 6, SLAU, 1, 4>;                        \
     case 6116: return &run_stage<R, S, 6, SLAU, 1, 6>;                        \
+    case 2200: return &run_stage<R, S, 2, ROE, 0, 0>;                         \
+    case 4200: return &run_stage<R, S, 4, ROE, 0, 0>;                         \
+    case 6200: return &run_stage<R, S, 6, ROE, 0, 0>;                         \
+    case 4214: return &run_stage<R, S, 4, ROE, 1, 4>;                         \
+    case 6216: return &run_stage<R, S, 6, ROE, 1, 6>;                         \
+    case 2300: return &run_stage<R, S, 2, HYBRID, 0, 0>;                      \
+    case 4300: return &run_stage<R, S, 4, HYBRID, 0, 0>;                      \
+    case 6300: return &run_stage<R, S, 6, HYBRID, 0, 0>;                      \
+    case 4314: return &run_stage<R, S, 4, HYBRID, 1, 4>;                      \
+    case 6314: return &run_stage<R, S, 6, HYBRID, 1, 4>;                      \
+    case 6316: return &run_stage<R, S, 6, HYBRID, 1, 6>;                      \
     default: return nullptr;                                                  \
   }
 

@@ -12,7 +12,7 @@ namespace ox {
 // as src/mod_constant.f90.fypp:18-35 writes them (and the module constants
 // of 3D_solver/src/calc_keep_kernel_internal.f90.fypp:7-12).
 struct HostConsts {
-  double gamma, gamma_1, R_over_gamma_1, Rgas, Cp, Cp_over_Pr, Prt, mu0c;
+  double gamma, gamma_1, over_gamma_1, R_over_gamma_1, Rgas, Cp, Cp_over_Pr, Prt, mu0c;
   double one_third, two_third, one_sixth, one_twelfth, one_sixty;
   double one_24, one_48, one_60, one_120, one_240, seven_twelfth;
   double gamma_over_gamma_1, minmod_b, threshold;
@@ -20,6 +20,7 @@ struct HostConsts {
   void init(const ox_config& c) {
     gamma = c.gamma;
     gamma_1 = c.gamma - 1.0;
+    over_gamma_1 = 1.0 / (c.gamma - 1.0);
     R_over_gamma_1 = c.R / (c.gamma - 1.0);
     Rgas = c.R;
     Cp = c.gamma * c.R / (c.gamma - 1.0);

@@ -40,6 +40,8 @@ def lib():
         L.oxo_keep_flux.argtypes = [ctypes.POINTER(_abi.OxConfig), ctypes.c_int, D, ctypes.c_int, D]
         L.oxo_slau_flux.argtypes = [ctypes.POINTER(_abi.OxConfig), ctypes.c_int, D, ctypes.c_int,
                                     ctypes.c_double, D]
+        L.oxo_conv_flux.argtypes = [ctypes.POINTER(_abi.OxConfig), ctypes.c_int, D, ctypes.c_int,
+                                    ctypes.c_double, D]
         L.oxo_muscl.argtypes = [ctypes.POINTER(_abi.OxConfig), ctypes.c_int, ctypes.c_int, ctypes.c_double, D,
                                 D, D]
         L.oxo_wiggle.argtypes = [D]

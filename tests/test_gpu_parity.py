@@ -42,6 +42,14 @@ CASES = [
     (6, "SLAU", "NS", 6, "hybrid", "HRSLAU2", 3),
     (6, "SLAU", "NS", 6, "tvd", "SLAU", 3),
     (6, "SLAU", "NS", 4, "hybrid", "HRSLAU2", 4),
+    (2, "Roe", "Euler", 0, "none", "HRSLAU2", 3),
+    (4, "Roe", "Euler", 0, "tvd", "HRSLAU2", 3),
+    (6, "Roe", "NS", 6, "hybrid", "HRSLAU2", 3),
+    (4, "Roe", "NS", 4, "none", "HRSLAU2", 4),
+    (2, "Hybrid", "Euler", 0, "none", "SLAU", 3),
+    (4, "Hybrid", "NS", 4, "tvd", "HRSLAU2", 4),
+    (6, "Hybrid", "NS", 6, "hybrid", "HRSLAU2", 3),
+    (6, "Hybrid", "Euler", 0, "hybrid", "SLAU", 3),
 ]
 
 

@@ -24,7 +24,6 @@ bool Oracle::init(const ox_config& cf) {
   vorder = cf.visc_order ? cf.visc_order : order;
   if (cf.visc != OX_VISC_EULER && vorder != 4 && vorder != 6) { err = "visc_order must be 4 or 6"; return false; }
   if (cf.visc != OX_VISC_EULER && vorder == 6 && order < 6) { err = "visc_order 6 needs order 6"; return false; }
-  if (cf.visc == OX_VISC_LES) { err = "LES is not in the oracle"; return false; }
   if (cf.scheme < OX_SCHEME_KEEP || cf.scheme > OX_SCHEME_HYBRID) { err = "unknown scheme"; return false; }
   if (cf.visc_stencil != OX_STENCIL_CENTRAL) { err = "ME4Base not in the oracle"; return false; }
   if (nx < 2 * order + 2 || ny < 2 * order + 2 || nz < 2 * order + 2) { err = "grid too small"; return false; }
@@ -68,6 +67,19 @@ bool Oracle::init(const ox_config& cf) {
   T.assign(ncell(), 0.0);
   mu.assign(ncell(), 0.0);
   sensor.assign(ncell(), 0.0);
+  mut.assign(ncell(), 0.0);
+  qc2.assign(ncell(), 0.0);
+  {
+    // SMS filter width (calc_les.f90:83-85) with dx = 1/spacing; the (1+alpha)
+    // power of it is a constant of the grid
+    const double idx = xix[0], idy = etay[0], idz = zetaz[0];
+    const double mn = std::min(idx, std::min(idy, idz));
+    const double a1 = mn / idx, a2 = mn / idy;
+    const double la1 = std::log(a1), la2 = std::log(a2);
+    const double delta = std::cosh(std::sqrt(4.0 * (la1 * la1 - la1 * la2 + la2 * la2) / 27.0)) *
+                         std::pow(idx * idy * idz, -1.0 / 3.0);
+    les_delta15 = std::pow(delta, 1.0 + 0.5);
+  }
   E.assign((size_t)5 * (nx - 1) * (ny - 2) * (nz - 2), 0.0);
   F.assign((size_t)5 * (nx - 2) * (ny - 1) * (nz - 2), 0.0);
   G.assign((size_t)5 * (nx - 2) * (ny - 2) * (nz - 1), 0.0);
@@ -292,7 +304,8 @@ static void visc_face(const Oracle& o, int dir, int i, int j, int k, double* tau
   const Consts& c = o.c;
   const int N = o.vorder;
   const int lo = (N == 4) ? -1 : -2;
-  double mu_[6], T_[6], u_[6], v_[6], w_[6], d1[6], d2[6], d3[6], d4[6];
+  const bool les = o.cfg.visc == OX_VISC_LES;
+  double mu_[6], T_[6], u_[6], v_[6], w_[6], d1[6], d2[6], d3[6], d4[6], mt[6], H[6];
   for (int m = 0; m < N; ++m) {
     const int s = lo + m;
     const int ii = i + (dir == 0 ? s : 0), jj = j + (dir == 1 ? s : 0), kk = k + (dir == 2 ? s : 0);
@@ -301,6 +314,11 @@ static void visc_face(const Oracle& o, int dir, int i, int j, int k, double* tau
     u_[m] = o.Qp[o.iq(ii, 2, jj, kk)];
     v_[m] = o.Qp[o.iq(ii, 3, jj, kk)];
     w_[m] = o.Qp[o.iq(ii, 4, jj, kk)];
+    if (les) {
+      mt[m] = o.mut[o.i3(ii, jj, kk)];
+      // H = Cp*T + 0.5*|u|^2 + qc2 (calc_visc_high_internal.f90.fypp:196-198)
+      H[m] = c.Cp * T_[m] + 0.5 * (u_[m] * u_[m] + v_[m] * v_[m] + w_[m] * w_[m]) + o.qc2[o.i3(ii, jj, kk)];
+    }
     if (dir == 0) {  // uy, vy, uz, wz
       d1[m] = cross_deriv(o, 1, 2, ii, jj, kk);
       d2[m] = cross_deriv(o, 1, 3, ii, jj, kk);
@@ -321,28 +339,36 @@ static void visc_face(const Oracle& o, int dir, int i, int j, int k, double* tau
   const double d = dir == 0 ? o.xix[i - 1] : (dir == 1 ? o.etay[j - 1] : o.zetaz[k - 1]);
   const double kT = c.Cp_over_Pr * vinterp(N, mu_) * vdiff(c, N, d, T_);
   double ta, tb, tc, ua, ub, uc;
+  auto straight = [&](const double* vel, const double* a, const double* b, double& t, double& ut) {
+    if (les) tau_straight_les(c, N, mu_, mt, vel, a, b, d, t, ut);
+    else tau_straight(c, N, mu_, vel, a, b, d, t, ut);
+  };
+  auto cross = [&](const double* vel, const double* g, double& t, double& ut) {
+    if (les) tau_cross_les(c, N, mu_, mt, vel, g, d, t, ut);
+    else tau_cross(c, N, mu_, vel, g, d, t, ut);
+  };
+  // SGS heat flux -interp(mut)*diff(H)/Prt (calc_visc_high_internal.f90.fypp:199)
+  const double Hsgs = les ? -vinterp(N, mt) * vdiff(c, N, d, H) / c.Prt : 0.0;
   if (dir == 0) {
     // txx, txy, txz
-    tau_straight(c, N, mu_, u_, d2 /*vy*/, d4 /*wz*/, d, ta, ua);
-    tau_cross(c, N, mu_, v_, d1 /*uy*/, d, tb, ub);
-    tau_cross(c, N, mu_, w_, d3 /*uz*/, d, tc, uc);
+    straight(u_, d2 /*vy*/, d4 /*wz*/, ta, ua);
+    cross(v_, d1 /*uy*/, tb, ub);
+    cross(w_, d3 /*uz*/, tc, uc);
     tau[0] = ta; tau[1] = tb; tau[2] = tc;
-    energy = ua + ub + uc + kT;
   } else if (dir == 1) {
     // tyy (straight), tyx, tyz
-    tau_straight(c, N, mu_, v_, d4 /*wz*/, d1 /*ux*/, d, tb, ub);
-    tau_cross(c, N, mu_, u_, d2 /*vx*/, d, ta, ua);
-    tau_cross(c, N, mu_, w_, d3 /*vz*/, d, tc, uc);
+    straight(v_, d4 /*wz*/, d1 /*ux*/, tb, ub);
+    cross(u_, d2 /*vx*/, ta, ua);
+    cross(w_, d3 /*vz*/, tc, uc);
     tau[0] = ta; tau[1] = tb; tau[2] = tc;
-    energy = ua + ub + uc + kT;
   } else {
     // tzz (straight), tzx, tzy
-    tau_straight(c, N, mu_, w_, d1 /*ux*/, d3 /*vy*/, d, tc, uc);
-    tau_cross(c, N, mu_, u_, d2 /*wx*/, d, ta, ua);
-    tau_cross(c, N, mu_, v_, d4 /*wy*/, d, tb, ub);
+    straight(w_, d1 /*ux*/, d3 /*vy*/, tc, uc);
+    cross(u_, d2 /*wx*/, ta, ua);
+    cross(v_, d4 /*wy*/, tb, ub);
     tau[0] = ta; tau[1] = tb; tau[2] = tc;
-    energy = ua + ub + uc + kT;
   }
+  energy = les ? ua + ub + uc + kT + Hsgs : ua + ub + uc + kT;
 }
 
 void Oracle::visc_x() {
@@ -396,6 +422,124 @@ void Oracle::visc_z() {
       }
 }
 
+// ---------------------------------------------------------------------------
+// Selective mixed-scale SGS model.  calc_les.f90:12-125; boundary layers by
+// set_bc_mut_common (3D_solver/src/set_bc_common.f90:296-335).
+// The exponents alpha/2 = (1-alpha)/2 = 1/4 are applied as sqrt(sqrt(.)).
+// ---------------------------------------------------------------------------
+static double test_filter(const double x[3][3][3]) {  // x[a][b][c] = Fortran x(a+1,b+1,c+1)
+  return x[1][1][1] + 0.25 * (x[0][1][1] - 2.0 * x[1][1][1] + x[2][1][1] + x[1][0][1] - 2.0 * x[1][1][1] +
+                              x[1][2][1] + x[1][1][0] - 2.0 * x[1][1][1] + x[1][1][2]);
+}
+
+static void differential(const double u[3][3][3], const double v[3][3][3], const double w[3][3][3], double dx,
+                         double dy, double dz, double* g) {
+  g[0] = 0.5 * (-u[0][1][1] + u[2][1][1]) * dx;  // dudx
+  g[1] = 0.5 * (-u[1][0][1] + u[1][2][1]) * dy;  // dudy
+  g[2] = 0.5 * (-u[1][1][0] + u[1][1][2]) * dz;  // dudz
+  g[3] = 0.5 * (-v[0][1][1] + v[2][1][1]) * dx;
+  g[4] = 0.5 * (-v[1][0][1] + v[1][2][1]) * dy;
+  g[5] = 0.5 * (-v[1][1][0] + v[1][1][2]) * dz;
+  g[6] = 0.5 * (-w[0][1][1] + w[2][1][1]) * dx;
+  g[7] = 0.5 * (-w[1][0][1] + w[1][2][1]) * dy;
+  g[8] = 0.5 * (-w[1][1][0] + w[1][1][2]) * dz;
+}
+
+static double sms(const Consts& c, const double u[3][3][3], const double v[3][3][3], const double w[3][3][3],
+                  const double ut[3][3][3], const double vt[3][3][3], const double wt[3][3][3], double dx, double dy,
+                  double dz, double qc2, double delta15) {
+  double g[9], h[9];
+  differential(u, v, w, dx, dy, dz, g);
+  differential(ut, vt, wt, dx, dy, dz, h);
+  const double vor1 = g[7] - g[5], vor2_ = g[2] - g[6], vor3 = g[3] - g[1];
+  const double vh1 = h[7] - h[5], vh2 = h[2] - h[6], vh3 = h[3] - h[1];
+  const double d1 = vor1 - vh1, d2 = vor2_ - vh2, d3 = vor3 - vh3;
+  const double vor2 = vor1 * vor1 + vor2_ * vor2_ + vor3 * vor3;
+  const double vorh2 = vh1 * vh1 + vh2 * vh2 + vh3 * vh3;
+  const double vdiff2 = d1 * d1 + d2 * d2 + d3 * d3;
+  const double svor2 = 2.0 * fsqrt(vor2) * fsqrt(vorh2) + vorh2 + vor2 - vdiff2;
+  double tan2;
+  if (fabs_(svor2) < 1.0e-30) tan2 = 0.0;
+  else tan2 = (2.0 * fsqrt(vorh2) * fsqrt(vor2) - vorh2 - vor2 + vdiff2) / svor2;
+  const double rtheta = tan2 / c.tan10deg2;
+  const double ftheta = fmin2(1.0, rtheta * rtheta);
+  const double S2 = 2.0 * (g[0] * g[0] + g[4] * g[4] + g[8] * g[8]) +
+                    ((g[3] + g[1]) * (g[3] + g[1]) + (g[7] + g[5]) * (g[7] + g[5]) + (g[2] + g[6]) * (g[2] + g[6]));
+  const double Cm = 0.06;
+  return ftheta * Cm * fsqrt(fsqrt(S2)) * fsqrt(fsqrt(qc2)) * delta15;
+}
+
+void Oracle::calc_mut() {
+  const std::vector<double>& Q = Qp;
+#pragma omp parallel for schedule(static)
+  for (int k = 2; k <= nz - 1; ++k)
+    for (int j = 2; j <= ny - 1; ++j)
+      for (int i = 2; i <= nx - 1; ++i) {
+        double u3[3][3][3], v3[3][3][3], w3[3][3][3], ut[3][3][3], vt[3][3][3], wt[3][3][3];
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b)
+            for (int cc = 0; cc < 3; ++cc) {
+              u3[a][b][cc] = Q[iq(i - 1 + a, 2, j - 1 + b, k - 1 + cc)];
+              v3[a][b][cc] = Q[iq(i - 1 + a, 3, j - 1 + b, k - 1 + cc)];
+              w3[a][b][cc] = Q[iq(i - 1 + a, 4, j - 1 + b, k - 1 + cc)];
+            }
+        if (3 <= i && i <= nx - 2 && 3 <= j && j <= ny - 2 && 3 <= k && k <= nz - 2) {
+          // stride_filter of the 5^3 neighbourhood (:88-94)
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+              for (int cc = 0; cc < 3; ++cc) {
+                double xu[3][3][3], xv[3][3][3], xw[3][3][3];
+                for (int p = 0; p < 3; ++p)
+                  for (int q = 0; q < 3; ++q)
+                    for (int r = 0; r < 3; ++r) {
+                      const int ii = i - 2 + a + p, jj = j - 2 + b + q, kk = k - 2 + cc + r;
+                      xu[p][q][r] = Q[iq(ii, 2, jj, kk)];
+                      xv[p][q][r] = Q[iq(ii, 3, jj, kk)];
+                      xw[p][q][r] = Q[iq(ii, 4, jj, kk)];
+                    }
+                ut[a][b][cc] = test_filter(xu);
+                vt[a][b][cc] = test_filter(xv);
+                wt[a][b][cc] = test_filter(xw);
+              }
+        } else {
+          std::memcpy(ut, u3, sizeof(u3));
+          std::memcpy(vt, v3, sizeof(v3));
+          std::memcpy(wt, w3, sizeof(w3));
+        }
+        const double du = u3[1][1][1] - ut[1][1][1], dv = v3[1][1][1] - vt[1][1][1], dw = w3[1][1][1] - wt[1][1][1];
+        const double q2 = 0.5 * (du * du + dv * dv + dw * dw);
+        qc2[i3(i, j, k)] = q2;
+        mut[i3(i, j, k)] = Q[iq(i, 1, j, k)] *
+                           sms(c, u3, v3, w3, ut, vt, wt, xix[i - 1], etay[j - 1], zetaz[k - 1], q2, les_delta15);
+      }
+  // set_bc_mut_common: x faces, y faces, xy corners, then z planes
+  auto bc = [&](std::vector<double>& a) {
+    for (int k = 2; k <= nz - 1; ++k)
+      for (int j = 2; j <= ny - 1; ++j) {
+        a[i3(1, j, k)] = a[i3(nx - 1, j, k)];
+        a[i3(nx, j, k)] = a[i3(2, j, k)];
+      }
+    for (int k = 2; k <= nz - 1; ++k)
+      for (int i = 2; i <= nx - 1; ++i) {
+        a[i3(i, 1, k)] = a[i3(i, ny - 1, k)];
+        a[i3(i, ny, k)] = a[i3(i, 2, k)];
+      }
+    for (int k = 2; k <= nz - 1; ++k) {
+      a[i3(1, 1, k)] = a[i3(nx - 1, ny - 1, k)];
+      a[i3(nx, 1, k)] = a[i3(2, ny - 1, k)];
+      a[i3(1, ny, k)] = a[i3(nx - 1, 2, k)];
+      a[i3(nx, ny, k)] = a[i3(2, 2, k)];
+    }
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        a[i3(i, j, 1)] = a[i3(i, j, nz - 1)];
+        a[i3(i, j, nz)] = a[i3(i, j, 2)];
+      }
+  };
+  bc(mut);
+  bc(qc2);
+}
+
 // calc_EFG (3D_solver/src/calc_flux_base.f90.fypp:109-198), periodic, no COMMZ
 void Oracle::calc_EFG(const std::vector<double>& q) {
   primitives(q);
@@ -403,6 +547,7 @@ void Oracle::calc_EFG(const std::vector<double>& q) {
   conv_x();
   conv_y();
   conv_z();
+  if (cfg.visc == OX_VISC_LES) calc_mut();
   if (cfg.visc != OX_VISC_EULER) {
     visc_x();
     visc_y();

@@ -418,6 +418,29 @@ inline void tau_cross(const Consts& c, int N, const double* mu, const double* v,
   t12 = mui * (uyi + vx);
   vt12 = vinterp(N, v) * t12;
 }
+// LES variants (calc_visc_cent.f90.fypp:44-58, :76-91): the work term uses the
+// molecular stress, the stress itself 0.5*(mu + mut)
+inline void tau_straight_les(const Consts& c, int N, const double* mu, const double* mut, const double* u,
+                             const double* vy, const double* wz, double d, double& t11, double& ut11) {
+  double mui = vinterp(N, mu);
+  const double vyi = vinterp(N, vy);
+  const double wzi = vinterp(N, wz);
+  const double ux = vdiff(c, N, d, u);
+  t11 = c.two_third * mui * (2.0 * ux - vyi - wzi);
+  ut11 = vinterp(N, u) * t11;
+  mui = 0.5 * (mui + vinterp(N, mut));
+  t11 = c.two_third * mui * (2.0 * ux - vyi - wzi);
+}
+inline void tau_cross_les(const Consts& c, int N, const double* mu, const double* mut, const double* v,
+                          const double* uy, double d, double& t12, double& vt12) {
+  double mui = vinterp(N, mu);
+  const double uyi = vinterp(N, uy);
+  const double vx = vdiff(c, N, d, v);
+  t12 = mui * (uyi + vx);
+  vt12 = vinterp(N, v) * t12;
+  mui = 0.5 * (mui + vinterp(N, mut));
+  t12 = mui * (uyi + vx);
+}
 
 // ---------------------------------------------------------------------------
 // The solver state.
@@ -431,7 +454,8 @@ struct Oracle {
   std::vector<double> Jac;               // (nx,ny)
   std::vector<double> dtdxdy, dtdydz, dtdzdx;
   std::vector<double> QJ, QJ2, Rs;       // QJ2 is QJs for RK4
-  std::vector<double> Qp, T, mu, sensor;
+  std::vector<double> Qp, T, mu, sensor, mut, qc2;
+  double les_delta15 = 0.0;  // delta**(1+alpha) of SMS (calc_les.f90:83-85), grid constant
   std::vector<double> E, F, G;
   std::string err;
 
@@ -464,6 +488,7 @@ struct Oracle {
   void ducros();                                   // calc_Ducros
   void conv_x(); void conv_y(); void conv_z();     // calc_<scheme>_{x,y,z}_in
   void visc_x(); void visc_y(); void visc_z();     // calc_{E,F,G}v{4,6}[_in]
+  void calc_mut();                                 // calc_les.f90:96-125 + set_bc_mut_common
   void calc_EFG(const std::vector<double>& q);     // calc_flux_base.f90.fypp:109-198
   void calc_R(int i, int j, int k, double cxy, double cyz, double czx, double* R) const;
   void step1(double coef, const std::vector<double>& Q, std::vector<double>& Q2);

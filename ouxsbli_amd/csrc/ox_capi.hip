@@ -130,7 +130,6 @@ int validate(const ox_config& c) {
   if (c.order != 2 && c.order != 4 && c.order != 6) return fail(OX_EINVAL, "order must be 2, 4 or 6");
   if (c.rk != 3 && c.rk != 4) return fail(OX_EINVAL, "rk must be 3 or 4");
   if (c.bc_x || c.bc_y || c.bc_z) return fail(OX_EUNSUPPORTED, "only periodic boundaries are built");
-  if (c.visc == OX_VISC_LES) return fail(OX_EUNSUPPORTED, "LES is not built yet");
   if (c.visc_stencil != OX_STENCIL_CENTRAL && c.visc != OX_VISC_EULER)
     return fail(OX_EUNSUPPORTED, "ME4Base viscous stencil is not built yet");
   if (c.scheme < OX_SCHEME_KEEP || c.scheme > OX_SCHEME_HYBRID) return fail(OX_EINVAL, "unknown scheme");
@@ -174,7 +173,7 @@ void destroy(ox_solver* s) {
   hipFree(s->Rs);
   hipFree(s->scratch);
   hipFree(s->C.Racc);
-  for (int m = 0; m < 8; ++m) hipFree(s->C.prim[m]);
+  for (int m = 0; m < 10; ++m) hipFree(s->C.prim[m]);
   for (hipEvent_t e : s->prof.pool) hipEventDestroy(e);
   if (s->ev0) hipEventDestroy(s->ev0);
   if (s->ev1) hipEventDestroy(s->ev1);
@@ -235,11 +234,12 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   C.cfg = *cfg;
   C.g = make_geom(*cfg);
   C.hc.init(*cfg);
+  C.hc.les_delta15 = ox::sms_delta15(C.g.inv_dx, C.g.inv_dy, C.g.inv_dz);
   C.vorder = cfg->visc == OX_VISC_EULER ? 0 : (cfg->visc_order ? cfg->visc_order : cfg->order);
   C.tvd = cfg->tvd;
   C.variant = cfg->slau_variant;
   C.Racc = nullptr;
-  for (int m = 0; m < 8; ++m) C.prim[m] = nullptr;
+  for (int m = 0; m < 10; ++m) C.prim[m] = nullptr;
   C.stream = nullptr;
   C.prof = false;
   C.mark = &prof_record;
@@ -270,7 +270,8 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   ok = ok && hipMalloc(&s->QJ2, ns * sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc(&C.Racc, ns * sizeof(double)) == hipSuccess;
   if (cfg->rk == 4) ok = ok && hipMalloc(&s->Rs, ns * sizeof(double)) == hipSuccess;
-  for (int m = 0; m < 8 && ok; ++m) ok = hipMalloc(&C.prim[m], nc * pe) == hipSuccess;
+  const int nprim = cfg->visc == OX_VISC_LES ? 10 : 8;
+  for (int m = 0; m < nprim && ok; ++m) ok = hipMalloc(&C.prim[m], nc * pe) == hipSuccess;
   s->scratch_bytes = 4096 * (2 * ox::DIAG_NQ + 1) * sizeof(double);
   ok = ok && hipMalloc(&s->scratch, s->scratch_bytes) == hipSuccess;
   ok = ok && hipEventCreate(&s->ev0) == hipSuccess && hipEventCreate(&s->ev1) == hipSuccess;
@@ -283,7 +284,7 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   ok = ok && hipMemsetAsync(s->QJ2, 0, ns * sizeof(double), C.stream) == hipSuccess;
   ok = ok && hipMemsetAsync(C.Racc, 0, ns * sizeof(double), C.stream) == hipSuccess;
   if (s->Rs) ok = ok && hipMemsetAsync(s->Rs, 0, ns * sizeof(double), C.stream) == hipSuccess;
-  for (int m = 0; m < 8 && ok; ++m) ok = hipMemsetAsync(C.prim[m], 0, nc * pe, C.stream) == hipSuccess;
+  for (int m = 0; m < nprim && ok; ++m) ok = hipMemsetAsync(C.prim[m], 0, nc * pe, C.stream) == hipSuccess;
   ok = ok && hipStreamSynchronize(C.stream) == hipSuccess;
   if (!ok) {
     destroy(s);

@@ -16,7 +16,7 @@ struct Core {
   int vorder;        // viscous order actually used (0 for Euler)
   int tvd, variant;  // runtime switches (uniform across the launch)
   // primitive arrays (element type per precision mode) and the accumulator
-  void* prim[8];     // rho,u,v,w,p,T,mu,sensor
+  void* prim[10];    // rho,u,v,w,p,T,mu,sensor,mut,qc2
   double* Racc;
   hipStream_t stream;
   // profiling: when `prof` is set, mark() records an event after a kernel

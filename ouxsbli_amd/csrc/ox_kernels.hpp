@@ -25,7 +25,7 @@
 namespace ox {
 
 template <class S> struct Prim {
-  S *rho, *u, *v, *w, *p, *T, *mu, *s;
+  S *rho, *u, *v, *w, *p, *T, *mu, *s, *mut, *qc2;
 };
 
 constexpr int KEEP = OX_SCHEME_KEEP;
@@ -85,10 +85,136 @@ __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ QJ, Pri
 }
 
 // ---------------------------------------------------------------------------
+// Selective mixed-scale SGS viscosity (calc_mut, 3D_solver/src/calc_les.f90:
+// 12-125) over the cells the reference computes (Fortran 2..n-1).  Only the
+// seven test-filtered values the central differences and qc2 read are formed
+// (the centre and its six face neighbours); exponents 1/4 as sqrt(sqrt(.)).
+// ---------------------------------------------------------------------------
+template <class S, class R>
+__device__ __forceinline__ R tfilt(const S* __restrict__ f, int m, int sy, int sz) {
+  const R c0 = R(f[m]);
+  return c0 + R(0.25) * (R(f[m - 1]) - R(2) * c0 + R(f[m + 1]) + R(f[m - sy]) - R(2) * c0 + R(f[m + sy]) +
+                         R(f[m - sz]) - R(2) * c0 + R(f[m + sz]));
+}
+
+template <class R>
+__device__ __forceinline__ void grad9(R um, R up, R vm, R vp, R wm, R wp, R dx, R dy, R dz, const R* uy, const R* vy,
+                                      const R* wy, const R* uz, const R* vz, const R* wz, R* g) {
+  g[0] = R(0.5) * (-um + up) * dx;        // dudx
+  g[1] = R(0.5) * (-uy[0] + uy[1]) * dy;  // dudy
+  g[2] = R(0.5) * (-uz[0] + uz[1]) * dz;  // dudz
+  g[3] = R(0.5) * (-vm + vp) * dx;
+  g[4] = R(0.5) * (-vy[0] + vy[1]) * dy;
+  g[5] = R(0.5) * (-vz[0] + vz[1]) * dz;
+  g[6] = R(0.5) * (-wm + wp) * dx;
+  g[7] = R(0.5) * (-wy[0] + wy[1]) * dy;
+  g[8] = R(0.5) * (-wz[0] + wz[1]) * dz;
+}
+
+template <class S, class R>
+__global__ __launch_bounds__(256) void k_les_mut(Prim<S> P, Geom g, PhysC<R> c) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.nx - 2 || j > g.ny - 2) return;
+  const int sy = g.nx, sz = g.nx * g.ny;
+  const int m = cidx(g, i, j, k);
+  const bool filt = (i >= 2 && i <= g.nx - 3 && j >= 2 && j <= g.ny - 3 && k >= 2 && k <= g.nz - 3);
+  const S* U[3] = {P.u, P.v, P.w};
+  R raw[3][7], fil[3][7];  // centre, -x, +x, -y, +y, -z, +z
+  const int off[7] = {0, -1, 1, -sy, sy, -sz, sz};
+#pragma unroll
+  for (int f = 0; f < 3; ++f)
+#pragma unroll
+    for (int t = 0; t < 7; ++t) {
+      raw[f][t] = R(U[f][m + off[t]]);
+      fil[f][t] = filt ? tfilt<S, R>(U[f], m + off[t], sy, sz) : raw[f][t];
+    }
+  const R dx = R(g.inv_dx), dy = R(g.inv_dy), dz = R(g.inv_dz);
+  R gr[9], gh[9];
+  {
+    const R uy[2] = {raw[0][3], raw[0][4]}, vy[2] = {raw[1][3], raw[1][4]}, wy[2] = {raw[2][3], raw[2][4]};
+    const R uz[2] = {raw[0][5], raw[0][6]}, vz[2] = {raw[1][5], raw[1][6]}, wz[2] = {raw[2][5], raw[2][6]};
+    grad9<R>(raw[0][1], raw[0][2], raw[1][1], raw[1][2], raw[2][1], raw[2][2], dx, dy, dz, uy, vy, wy, uz, vz, wz,
+             gr);
+  }
+  {
+    const R uy[2] = {fil[0][3], fil[0][4]}, vy[2] = {fil[1][3], fil[1][4]}, wy[2] = {fil[2][3], fil[2][4]};
+    const R uz[2] = {fil[0][5], fil[0][6]}, vz[2] = {fil[1][5], fil[1][6]}, wz[2] = {fil[2][5], fil[2][6]};
+    grad9<R>(fil[0][1], fil[0][2], fil[1][1], fil[1][2], fil[2][1], fil[2][2], dx, dy, dz, uy, vy, wy, uz, vz, wz,
+             gh);
+  }
+  const R du = raw[0][0] - fil[0][0], dv = raw[1][0] - fil[1][0], dw = raw[2][0] - fil[2][0];
+  const R q2 = R(0.5) * (du * du + dv * dv + dw * dw);
+  // SMS (calc_les.f90:30-86)
+  const R vor1 = gr[7] - gr[5], vor2_ = gr[2] - gr[6], vor3 = gr[3] - gr[1];
+  const R vh1 = gh[7] - gh[5], vh2 = gh[2] - gh[6], vh3 = gh[3] - gh[1];
+  const R d1 = vor1 - vh1, d2 = vor2_ - vh2, d3 = vor3 - vh3;
+  const R vor2 = vor1 * vor1 + vor2_ * vor2_ + vor3 * vor3;
+  const R vorh2 = vh1 * vh1 + vh2 * vh2 + vh3 * vh3;
+  const R vdiff2 = d1 * d1 + d2 * d2 + d3 * d3;
+  const R svor2 = R(2) * fsqrt(vor2) * fsqrt(vorh2) + vorh2 + vor2 - vdiff2;
+  R tan2;
+  if (fabs_(svor2) < R(1.0e-30)) tan2 = R(0);
+  else tan2 = (R(2) * fsqrt(vorh2) * fsqrt(vor2) - vorh2 - vor2 + vdiff2) / svor2;
+  const R rtheta = tan2 / c.tan10deg2;
+  const R ftheta = fmin2(R(1), rtheta * rtheta);
+  const R S2 = R(2) * (gr[0] * gr[0] + gr[4] * gr[4] + gr[8] * gr[8]) +
+               ((gr[3] + gr[1]) * (gr[3] + gr[1]) + (gr[7] + gr[5]) * (gr[7] + gr[5]) +
+                (gr[2] + gr[6]) * (gr[2] + gr[6]));
+  const R nut = ftheta * R(0.06) * fsqrt(fsqrt(S2)) * fsqrt(fsqrt(q2)) * c.les_delta15;
+  P.qc2[m] = (S)q2;
+  P.mut[m] = (S)(R(P.rho[m]) * nut);
+}
+
+// set_bc_mut_common (3D_solver/src/set_bc_common.f90:296-335): pass 0 = x
+// faces, y faces and xy corners over k = Fortran 2..nz-1; pass 1 = z planes.
+template <class S>
+__global__ __launch_bounds__(256) void k_les_bc(Prim<S> P, Geom g, int pass) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  S* A[2] = {P.mut, P.qc2};
+  if (pass == 0) {
+    const int nk = nz - 2;
+    const long long nxf = (long long)(ny - 2) * nk, nyf = (long long)(nx - 2) * nk;
+    if (t < nxf) {
+      const int j = 1 + (int)(t % (ny - 2)), k = 1 + (int)(t / (ny - 2));
+      for (int a = 0; a < 2; ++a) {
+        A[a][cidx(g, 0, j, k)] = A[a][cidx(g, nx - 2, j, k)];
+        A[a][cidx(g, nx - 1, j, k)] = A[a][cidx(g, 1, j, k)];
+      }
+    } else if (t < nxf + nyf) {
+      const long long u = t - nxf;
+      const int i = 1 + (int)(u % (nx - 2)), k = 1 + (int)(u / (nx - 2));
+      for (int a = 0; a < 2; ++a) {
+        A[a][cidx(g, i, 0, k)] = A[a][cidx(g, i, ny - 2, k)];
+        A[a][cidx(g, i, ny - 1, k)] = A[a][cidx(g, i, 1, k)];
+      }
+    } else if (t < nxf + nyf + nk) {
+      const int k = 1 + (int)(t - nxf - nyf);
+      for (int a = 0; a < 2; ++a) {
+        A[a][cidx(g, 0, 0, k)] = A[a][cidx(g, nx - 2, ny - 2, k)];
+        A[a][cidx(g, nx - 1, 0, k)] = A[a][cidx(g, 1, ny - 2, k)];
+        A[a][cidx(g, 0, ny - 1, k)] = A[a][cidx(g, nx - 2, 1, k)];
+        A[a][cidx(g, nx - 1, ny - 1, k)] = A[a][cidx(g, 1, 1, k)];
+      }
+    }
+  } else {
+    if (t < (long long)nx * ny) {
+      const int i = (int)(t % nx), j = (int)(t / nx);
+      for (int a = 0; a < 2; ++a) {
+        A[a][cidx(g, i, j, 0)] = A[a][cidx(g, i, j, nz - 2)];
+        A[a][cidx(g, i, j, nz - 1)] = A[a][cidx(g, i, j, 1)];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // One cell's worth of window data.
 // ---------------------------------------------------------------------------
 template <class R, int DIR, int ORDER, int SCHEME, int VISC, int VO> struct CellData {
-  R rho, u, v, w, p, T, mu, s, d1, d2, d3, d4;
+  R rho, u, v, w, p, T, mu, s, d1, d2, d3, d4, mt, H;  // mt, H: LES only
 };
 
 // cross derivative of field f along axis `ax` at cell m (stride st)
@@ -114,6 +240,10 @@ __device__ __forceinline__ void load_cell(const Prim<S>& P, const Geom& g, const
   if constexpr (LOAD_S && kUsesSensor<SCHEME, ORDER>) d.s = R(P.s[ms]);
   if constexpr (VISC) {
     d.mu = R(P.mu[m]);
+    if constexpr (VISC == 2) {
+      d.mt = R(P.mut[m]);
+      d.H = c.Cp * d.T + R(0.5) * (d.u * d.u + d.v * d.v + d.w * d.w) + R(P.qc2[m]);
+    }
     const int sx = 1, sy = g.nx, sz = g.nx * g.ny;
     if constexpr (DIR == 0) {  // uy, vy, uz, wz
       d.d1 = dcross<R, S, VO>(c, P.u, m, sy, R(g.inv_dy));
@@ -171,15 +301,16 @@ __device__ __forceinline__ void face_flux(const PhysC<R>& c, int tvd, int varian
   }
   if constexpr (VISC) {
     constexpr int M = VO;
-    R mu_[M], T_[M], u_[M], v_[M], w_[M], a1[M], a2[M], a3[M], a4[M];
+    R mu_[M], T_[M], u_[M], v_[M], w_[M], a1[M], a2[M], a3[M], a4[M], mt_[M], H_[M];
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       const auto& x = w[St::VOFF + m];
       mu_[m] = x.mu; T_[m] = x.T; u_[m] = x.u; v_[m] = x.v; w_[m] = x.w;
       a1[m] = x.d1; a2[m] = x.d2; a3[m] = x.d3; a4[m] = x.d4;
+      if constexpr (VISC == 2) { mt_[m] = x.mt; H_[m] = x.H; }
     }
     R tau[3], en;
-    visc_face<R, DIR, M>(c, inv_d, mu_, T_, u_, v_, w_, a1, a2, a3, a4, tau, en);
+    visc_face<R, DIR, M, (VISC == 2)>(c, inv_d, mu_, T_, u_, v_, w_, a1, a2, a3, a4, tau, en, mt_, H_);
     F[1] = F[1] - tau[0];
     F[2] = F[2] - tau[1];
     F[3] = F[3] - tau[2];
@@ -235,7 +366,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__
   using CD = CellData<R, 0, ORDER, SCHEME, VISC, VO>;
   constexpr bool SENS = kUsesSensor<SCHEME, ORDER>;
   constexpr int W = XT + St::SW;
-  constexpr int NF = 12;  // rho,u,v,w,p,T,mu,s,d1..d4
+  constexpr int NF = VISC == 2 ? 14 : 12;  // rho,u,v,w,p,T,mu,s,d1..d4[,mt,H]
   __shared__ R tile[NF][XROWS][W];
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = g.h + blockIdx.x * XT;
@@ -253,6 +384,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__
       if constexpr (VISC) {
         tile[6][ty][m] = d.mu;
         tile[8][ty][m] = d.d1; tile[9][ty][m] = d.d2; tile[10][ty][m] = d.d3; tile[11][ty][m] = d.d4;
+        if constexpr (VISC == 2) { tile[NF - 2][ty][m] = d.mt; tile[NF - 1][ty][m] = d.H; }
       }
     }
   }
@@ -291,6 +423,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__
         win[m].mu = tile[6][ty][tx + m];
         win[m].d1 = tile[8][ty][tx + m]; win[m].d2 = tile[9][ty][tx + m];
         win[m].d3 = tile[10][ty][tx + m]; win[m].d4 = tile[11][ty][tx + m];
+        if constexpr (VISC == 2) { win[m].mt = tile[NF - 2][ty][tx + m]; win[m].H = tile[NF - 1][ty][tx + m]; }
       }
     }
     face_flux<R, 0, ORDER, SCHEME, VISC, VO>(c, tvd, variant, R(g.inv_dx), win, F);

@@ -38,6 +38,7 @@ template <class R> struct PhysC {
   R one_third, two_third, one_sixth, one_twelfth, one_sixty;
   R one_24, one_48, one_60, one_120, one_240, seven_twelfth;
   R gamma_over_gamma_1, minmod_b, threshold;
+  R Cp, Prt, tan10deg2, les_delta15;
 };
 
 template <class R> __host__ PhysC<R> make_physc(const HostConsts& h) {
@@ -49,6 +50,7 @@ template <class R> __host__ PhysC<R> make_physc(const HostConsts& h) {
   c.one_24 = (R)h.one_24; c.one_48 = (R)h.one_48; c.one_60 = (R)h.one_60; c.one_120 = (R)h.one_120;
   c.one_240 = (R)h.one_240; c.seven_twelfth = (R)h.seven_twelfth;
   c.gamma_over_gamma_1 = (R)h.gamma_over_gamma_1; c.minmod_b = (R)h.minmod_b; c.threshold = (R)h.threshold;
+  c.Cp = (R)h.Cp; c.Prt = (R)h.Prt; c.tan10deg2 = (R)h.tan10deg2; c.les_delta15 = (R)h.les_delta15;
   return c;
 }
 
@@ -407,10 +409,13 @@ template <class R, int N> OX_HD R vdiff(const PhysC<R>& c, R d, const R* a) {
            d;
 }
 
-template <class R, int DIR, int N>
+// LES (calc_visc_cent.f90.fypp:44-58, :76-91; calc_visc_high_internal.f90.fypp:196-199):
+// mt = rho*nu_t at the stencil, H = Cp*T + |u|^2/2 + qc2; the stresses use
+// 0.5*(mu + mut), the work terms the molecular stress, plus the SGS heat flux.
+template <class R, int DIR, int N, bool LES = false>
 OX_HD void visc_face(const PhysC<R>& c, R d, const R* mu, const R* T, const R* u, const R* v,
                                           const R* w, const R* d1, const R* d2, const R* d3, const R* d4, R* tau,
-                                          R& energy) {
+                                          R& energy, const R* mt = nullptr, const R* H = nullptr) {
   const R mui = vinterp<R, N>(mu);
   const R kT = c.Cp_over_Pr * mui * vdiff<R, N>(c, d, T);
   // straight component: (vel_DIR, a, b); cross components: (vel_m, g_m)
@@ -420,17 +425,28 @@ OX_HD void visc_face(const PhysC<R>& c, R d, const R* mu, const R* T, const R* u
   const R ai = vinterp<R, N>(sa);
   const R bi = vinterp<R, N>(sb);
   const R dn = vdiff<R, N>(c, d, vs);
-  const R ts = c.two_third * mui * (R(2) * dn - ai - bi);
+  R ts = c.two_third * mui * (R(2) * dn - ai - bi);
   const R uts = vinterp<R, N>(vs) * ts;
   // the two cross components in component order
   const R* va = DIR == 0 ? v : u;                      // first cross velocity
   const R* ga = DIR == 0 ? d1 : d2;                    // uy | vx | wx
   const R* vb = DIR == 2 ? v : w;                      // second cross velocity
   const R* gb = DIR == 0 ? d3 : (DIR == 1 ? d3 : d4);  // uz | vz | wy
-  const R ta = mui * (vinterp<R, N>(ga) + vdiff<R, N>(c, d, va));
+  const R ga_i = vinterp<R, N>(ga), va_d = vdiff<R, N>(c, d, va);
+  const R gb_i = vinterp<R, N>(gb), vb_d = vdiff<R, N>(c, d, vb);
+  R ta = mui * (ga_i + va_d);
   const R uta = vinterp<R, N>(va) * ta;
-  const R tb = mui * (vinterp<R, N>(gb) + vdiff<R, N>(c, d, vb));
+  R tb = mui * (gb_i + vb_d);
   const R utb = vinterp<R, N>(vb) * tb;
+  R Hsgs = R(0);
+  if constexpr (LES) {
+    const R mti = vinterp<R, N>(mt);
+    const R m2 = R(0.5) * (mui + mti);
+    ts = c.two_third * m2 * (R(2) * dn - ai - bi);
+    ta = m2 * (ga_i + va_d);
+    tb = m2 * (gb_i + vb_d);
+    Hsgs = -(mti * vdiff<R, N>(c, d, H) / c.Prt);
+  }
   if constexpr (DIR == 0) {
     tau[0] = ts; tau[1] = ta; tau[2] = tb;
     energy = uts + uta + utb + kT;  // utxx + vtxy + wtxz + kTx
@@ -441,6 +457,7 @@ OX_HD void visc_face(const PhysC<R>& c, R d, const R* mu, const R* T, const R* u
     tau[0] = ta; tau[1] = tb; tau[2] = ts;
     energy = uta + utb + uts + kT;  // utzx + vtzy + wtzz + kTz
   }
+  if constexpr (LES) energy = energy + Hsgs;
 }
 
 // Cross derivative stencils of the LDS loaders

@@ -26,11 +26,20 @@ void run_stage(Core& C, const double* Qsrc, const StageArgs& sa, double cx, doub
   P.T = (S*)C.prim[5];
   P.mu = (S*)C.prim[6];
   P.s = (S*)C.prim[7];
+  P.mut = (S*)C.prim[8];
+  P.qc2 = (S*)C.prim[9];
   const PhysC<double> c64 = make_physc<double>(C.hc);
   const PhysC<R> cr = make_physc<R>(C.hc);
   const dim3 blk(64, 4, 1);
   k_prim<S, (VISC != 0)><<<dim3(cdiv(g.nx, 64), cdiv(g.ny, 4), g.nz), blk, 0, C.stream>>>(Qsrc, P, g, c64);
   prof_mark(C, "prim");
+  if constexpr (VISC == 2) {
+    k_les_mut<S, R><<<dim3(cdiv(g.nx - 2, 64), cdiv(g.ny - 2, 4), g.nz - 2), blk, 0, C.stream>>>(P, g, cr);
+    const long long n0 = (long long)(g.ny - 2) * (g.nz - 2) + (long long)(g.nx - 2) * (g.nz - 2) + (g.nz - 2);
+    k_les_bc<S><<<(unsigned)((n0 + 255) / 256), 256, 0, C.stream>>>(P, g, 0);
+    k_les_bc<S><<<(unsigned)(((long long)g.nx * g.ny + 255) / 256), 256, 0, C.stream>>>(P, g, 1);
+    prof_mark(C, "les_mut");
+  }
   k_sweep_x<R, S, ORDER, SCHEME, VISC, VO>
       <<<dim3(cdiv(g.ix, XT), cdiv(g.iy, XROWS), g.iz), blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd, C.variant, cx);
   prof_mark(C, "sweep_x");
@@ -84,6 +93,11 @@ void run_stage(Core& C, const double* Qsrc, const StageArgs& sa, double cx, doub
     case 4314: return &run_stage<R, S, 4, HYBRID, 1, 4>;                      \
     case 6314: return &run_stage<R, S, 6, HYBRID, 1, 4>;                      \
     case 6316: return &run_stage<R, S, 6, HYBRID, 1, 6>;                      \
+    case 4024: return &run_stage<R, S, 4, KEEP, 2, 4>;                        \
+    case 6026: return &run_stage<R, S, 6, KEEP, 2, 6>;                        \
+    case 6126: return &run_stage<R, S, 6, SLAU, 2, 6>;                        \
+    case 4324: return &run_stage<R, S, 4, HYBRID, 2, 4>;                      \
+    case 6326: return &run_stage<R, S, 6, HYBRID, 2, 6>;                      \
     default: return nullptr;                                                  \
   }
 

@@ -1,6 +1,7 @@
 // ox_types.hpp -- host/device shared plain types of the gfx950 library.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -16,6 +17,8 @@ struct HostConsts {
   double one_third, two_third, one_sixth, one_twelfth, one_sixty;
   double one_24, one_48, one_60, one_120, one_240, seven_twelfth;
   double gamma_over_gamma_1, minmod_b, threshold;
+  double tan10deg2;   // (tan 10 deg)^2, the SMS angle selector (mod_constant.f90.fypp:34)
+  double les_delta15; // SMS filter width to the power 1+alpha (calc_les.f90:83-85), set per grid
 
   void init(const ox_config& c) {
     gamma = c.gamma;
@@ -41,8 +44,26 @@ struct HostConsts {
     gamma_over_gamma_1 = c.gamma * (1.0 / (c.gamma - 1.0));
     minmod_b = (3.0 - one_third) / (1.0 - one_third);
     threshold = c.threshold;
+    const double t = std::tan(10.0 * std::acos(-1.0) / 180.0);
+    tan10deg2 = t * t;
+    // SMS filter width from the inverse spacings of a uniform periodic grid
+    const double idx = (double)(c.nx - c.order) / c.Lx, idy = (double)(c.ny - c.order) / c.Ly,
+                 idz = (double)(c.nz - c.order) / c.Lz;
+    (void)idx; (void)idy; (void)idz;
+    les_delta15 = 0.0;
   }
 };
+
+// SMS filter width to the power 1+alpha from the three inverse spacings
+// (calc_les.f90:83-85), evaluated on the host once per grid.
+inline double sms_delta15(double idx, double idy, double idz) {
+  const double mn = std::min(idx, std::min(idy, idz));
+  const double a1 = mn / idx, a2 = mn / idy;
+  const double la1 = std::log(a1), la2 = std::log(a2);
+  const double delta =
+      std::cosh(std::sqrt(4.0 * (la1 * la1 - la1 * la2 + la2 * la2) / 27.0)) * std::pow(idx * idy * idz, -1.0 / 3.0);
+  return std::pow(delta, 1.0 + 0.5);
+}
 
 // Grid geometry of a uniform periodic grid (set_grid_cyclic*, all spacings
 // equal).  Every metric array of the reference is constant, so the kernels

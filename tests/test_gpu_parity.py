@@ -50,6 +50,9 @@ CASES = [
     (4, "Hybrid", "NS", 4, "tvd", "HRSLAU2", 4),
     (6, "Hybrid", "NS", 6, "hybrid", "HRSLAU2", 3),
     (6, "Hybrid", "Euler", 0, "hybrid", "SLAU", 3),
+    (6, "Hybrid", "LES", 6, "hybrid", "HRSLAU2", 4),
+    (6, "SLAU", "LES", 6, "hybrid", "HRSLAU2", 3),
+    (4, "KEEP", "LES", 4, "none", "HRSLAU2", 4),
 ]
 
 

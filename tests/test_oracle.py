@@ -351,3 +351,50 @@ def test_viscous_shear_rate(vo, order, min_rate):
     e2 = _shear_error(48, vo, order)
     assert e1 < 1e-2
     assert math.log(e1 / e2, 2) >= min_rate, (e1, e2)
+
+
+def test_les_selective_model_switches_off_for_single_mode():
+    """The SMS angle selector (calc_les.f90:57-66) gives nu_t = 0 when the
+    test-filtered vorticity is parallel to the vorticity: for the single
+    Fourier mode of the TGV the LES RHS then differs from NS only through the
+    0.5*(mu + mut) stress factor (calc_visc_cent.f90.fypp:56), i.e. the
+    momentum viscous stress is halved and the energy equation is unchanged."""
+    kw = dict(Ma=0.3, cfl=0.3, cfl_mode="acoustic", scheme="KEEP", rk=4, tvd="none")
+    les = cases.tgv_spec(16, 6, visc="LES", **kw)
+    ns = cases.tgv_spec(16, 6, visc="NS", **kw)
+    eul = cases.tgv_spec(16, 6, visc="Euler", **kw)
+    Q = cases.tgv_state(ns)
+    R = {}
+    for name, s in (("les", les), ("ns", ns), ("eul", eul)):
+        o = Oracle(s)
+        o.set_state(Q)
+        R[name] = o.rhs()
+    np.testing.assert_array_equal(R["les"][:, :, 4], R["ns"][:, :, 4])
+    vis_ns = R["ns"][:, :, 1:4] - R["eul"][:, :, 1:4]
+    vis_les = R["les"][:, :, 1:4] - R["eul"][:, :, 1:4]
+    np.testing.assert_allclose(vis_les, 0.5 * vis_ns, rtol=1e-6, atol=1e-9 * np.abs(vis_ns).max())
+
+
+def test_les_active_for_multimode_field():
+    """For a perturbed (multi-mode) field nu_t > 0: the SGS heat flux changes
+    the energy RHS against NS.  The periodic sum is not exactly zero: the
+    reference fills the outer two ghost layers of mut with copies that are not
+    periodic images (set_bc_mut_common, set_bc_common.f90:296-335, after
+    calc_mut leaves cells 2 and n-1 unfiltered with qc2 = 0), so the seam
+    faces see different mut on the two sides; the imbalance stays small."""
+    kw = dict(Ma=0.3, cfl=0.3, cfl_mode="acoustic", scheme="Hybrid", rk=4, tvd="hybrid")
+    s = cases.tgv_spec(16, 6, visc="LES", **kw)
+    ns = cases.tgv_spec(16, 6, visc="NS", **kw)
+    Q = cases.tgv_state(s)
+    rng = np.random.default_rng(5)
+    h = 3
+    Q[h:-h, h:-h, 1:4, h:-h] += 0.2 * np.abs(Q[:, :, 1:3]).max() * rng.standard_normal(Q[h:-h, h:-h, 1:4, h:-h].shape)
+    Q = cases.periodic_fill(Q, 6)
+    o, on = Oracle(s), Oracle(ns)
+    o.set_state(Q)
+    on.set_state(Q)
+    R, Rn = o.rhs(), on.rhs()
+    assert np.abs(R[h:-h, h:-h, 4, h:-h] - Rn[h:-h, h:-h, 4, h:-h]).max() > 0
+    Ri = R[h:-h, h:-h, :, h:-h]
+    for l in range(5):
+        assert abs(Ri[:, :, l].sum()) <= 1e-4 * np.abs(Ri[:, :, l]).sum()

@@ -33,7 +33,8 @@ constexpr int SLAU = OX_SCHEME_SLAU;
 constexpr int ROE = OX_SCHEME_ROE;
 constexpr int HYBRID = OX_SCHEME_HYBRID;
 // schemes that read the Ducros sensor (calc_flux_base.f90.fypp:56-58)
-template <int SCHEME, int ORDER> constexpr bool kUsesSensor = SCHEME != KEEP && ORDER > 2;
+// (MUSCL reads it at ORDER 4/6; the hybrid switch at every order)
+template <int SCHEME, int ORDER> constexpr bool kUsesSensor = SCHEME == HYBRID || (SCHEME != KEEP && ORDER > 2);
 // schemes whose face flux reads T (KEEP's internal-energy term)
 template <int SCHEME> constexpr bool kUsesT = SCHEME == KEEP || SCHEME == HYBRID;
 
@@ -390,16 +391,21 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__
   }
   __syncthreads();
   if constexpr (SENS) {
-    // sensor at tile positions HW-1 .. HW+63 (the cells either side of the 64 faces)
+    // sensor at tile positions HW-1 .. HW+63 (the cells either side of the 64
+    // faces).  With one ghost layer (ORDER 2) the ghost cell's sensor is the
+    // boundary copy of its interior neighbour (calc_hybrid.f90:54-71); with
+    // more layers the ghost cells used are computed like interior ones.
     for (int m = St::HW - 1 + tx; m <= St::HW + 63; m += 64) {
       const int i = i0 - St::HW + m;
-      if (rowok && i < g.nx - 1) {
-        const int q = cidx(g, i, j, k);
+      if (rowok && i <= g.nx - 1) {
+        // clamping also keeps the unused positions past the last face in bounds
+        const int ic = min(max(i, 1), g.nx - 2);
+        const int q = cidx(g, ic, j, k);
         const int sy = g.nx, sz = g.nx * g.ny;
-        const R fd = ducros<R>(g, tile[1][ty][m - 1], tile[1][ty][m + 1], tile[2][ty][m - 1], tile[2][ty][m + 1],
-                               tile[3][ty][m - 1], tile[3][ty][m + 1], R(P.u[q - sy]), R(P.u[q + sy]), R(P.v[q - sy]),
-                               R(P.v[q + sy]), R(P.w[q - sy]), R(P.w[q + sy]), R(P.u[q - sz]), R(P.u[q + sz]),
-                               R(P.v[q - sz]), R(P.v[q + sz]), R(P.w[q - sz]), R(P.w[q + sz]));
+        const R fd = ducros<R>(g, R(P.u[q - 1]), R(P.u[q + 1]), R(P.v[q - 1]), R(P.v[q + 1]), R(P.w[q - 1]),
+                               R(P.w[q + 1]), R(P.u[q - sy]), R(P.u[q + sy]), R(P.v[q - sy]), R(P.v[q + sy]),
+                               R(P.w[q - sy]), R(P.w[q + sy]), R(P.u[q - sz]), R(P.u[q + sz]), R(P.v[q - sz]),
+                               R(P.v[q + sz]), R(P.w[q - sz]), R(P.w[q + sz]));
         tile[7][ty][m] = fd;
         if (i >= i0 && i < i0 + XT && i < g.h + g.ix) P.s[q] = (S)fd;  // each cell stored by one tile
       }
@@ -465,8 +471,13 @@ __global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, double* __restric
   const int qstride = DIR == 1 ? 5 * g.nx : 5 * g.nx * g.ny;
   const R inv_d = DIR == 1 ? R(g.inv_dy) : R(g.inv_dz);
   // the sensor is stored for interior cells only; a ghost cell's sensor is
-  // bitwise that of its periodic image (its inputs are exact copies)
-  auto wrap = [&](int sp) { return sp < g.h ? sp + nd : (sp >= g.h + nd ? sp - nd : sp); };
+  // bitwise that of its periodic image (its inputs are exact copies), except
+  // with one ghost layer (ORDER 2), where the reference copies the adjacent
+  // interior value into the ghost cell (calc_hybrid.f90:54-71)
+  auto wrap = [&](int sp) {
+    if constexpr (ORDER == 2) return sp < g.h ? g.h : (sp >= g.h + nd ? g.h + nd - 1 : sp);
+    return sp < g.h ? sp + nd : (sp >= g.h + nd ? sp - nd : sp);
+  };
   CD win[St::SW];
 #pragma unroll
   for (int m = 0; m < St::SW - 1; ++m) {

@@ -52,6 +52,19 @@ def kernel_model(spec, ops):
     return m
 
 
+# HBM bytes per launch of the FP64 kernels at 256^3, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this
+# script, scaled by the calibration a 1 GiB device copy gives (FETCH_SIZE x2, WRITE_SIZE x1):
+# profiles/hbm_traffic_tgv256_slau6_ns6_rk3.md.  Only valid for the kernels as they are built now; remeasure
+# after a kernel change.
+MEASURED_TRAFFIC_256 = {
+    "prim": 1957e6,
+    "sweep_x": 2936e6,
+    "sweep_y": 3504e6,
+    "sweep_z_rk": (4374e6 + 2 * 5214e6) / 3,
+    "ghost_fill": 176e6,
+}
+
+
 def cpu_baseline(spec_small, seconds=15.0):
     """The FP64 CPU oracle on a bounded sample of the same scheme."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -160,7 +173,8 @@ def main():
                       "speedup_vs_fp64": round(value32 / value, 3),
                       "ke_rel_err_vs_fp64": abs(ke32 - ke64) / abs(ke64)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC_256.get(dom) if args.n == 256 else None,
                      "alg_bytes_per_launch": model[dom]["bytes"], "avg_launch_ms": round(avg_ms, 4)},
         "stage_roofline": {"F_alg_per_cell_stage": F_alg, "B_alg_per_cell_stage": round(B_alg, 1),
                            "binding": "fp64" if FP64_PEAK_TFLOPS * 1e12 / F_alg < HBM_PEAK_GBS * 1e9 / B_alg else "hbm",

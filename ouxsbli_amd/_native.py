@@ -10,7 +10,10 @@ import pathlib
 
 from . import _abi
 
-LIB_PATH = pathlib.Path(__file__).resolve().parent / "libouxsbli.so"
+import os
+
+# OUXSBLI_LIB selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = pathlib.Path(os.environ.get("OUXSBLI_LIB") or pathlib.Path(__file__).resolve().parent / "libouxsbli.so")
 
 _lib = None
 

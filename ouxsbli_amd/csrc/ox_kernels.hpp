@@ -218,13 +218,96 @@ template <class R, int DIR, int ORDER, int SCHEME, int VISC, int VO> struct Cell
   R rho, u, v, w, p, T, mu, s, d1, d2, d3, d4, mt, H;  // mt, H: LES only
 };
 
-// cross derivative of field f along axis `ax` at cell m (stride st)
+// The values one window cell is built from, as they sit in memory: its
+// primitives and, for viscous runs, the transverse stencil values of the four
+// cross derivatives a face in direction DIR needs (centre excluded).  Kept
+// separate from CellData so that a sweep can issue the loads of the next cell
+// one face ahead of their use.
+template <class S, int VISC, int VO> struct RawCell {
+  static constexpr int NX = VISC ? VO : 1;
+  S rho, u, v, w, p, T, mu, s, mt, qc2;
+  S x[4][NX];
+};
+
+// The four cross derivatives by face direction:
+//   DIR 0: uy, vy, uz, wz   DIR 1: ux, vx, vz, wz   DIR 2: ux, wx, vy, wy
+template <class S, int DIR>
+__device__ __forceinline__ void cross_fields(const Prim<S>& P, const Geom& g, const S* (&f)[4], int (&st)[4]) {
+  const int sx = 1, sy = g.nx, sz = g.nx * g.ny;
+  if constexpr (DIR == 0) {
+    f[0] = P.u; st[0] = sy; f[1] = P.v; st[1] = sy; f[2] = P.u; st[2] = sz; f[3] = P.w; st[3] = sz;
+  } else if constexpr (DIR == 1) {
+    f[0] = P.u; st[0] = sx; f[1] = P.v; st[1] = sx; f[2] = P.v; st[2] = sz; f[3] = P.w; st[3] = sz;
+  } else {
+    f[0] = P.u; st[0] = sx; f[1] = P.w; st[1] = sx; f[2] = P.v; st[2] = sy; f[3] = P.w; st[3] = sy;
+  }
+}
+
+// Issues the loads of cell m (sensor read at ms, its periodic image).
+template <class S, int DIR, int ORDER, int SCHEME, int VISC, int VO, bool LOAD_S = true>
+__device__ __forceinline__ void fetch_cell(const Prim<S>& P, const Geom& g, int m, int ms, RawCell<S, VISC, VO>& r) {
+  r.rho = P.rho[m];
+  r.u = P.u[m];
+  r.v = P.v[m];
+  r.w = P.w[m];
+  r.p = P.p[m];
+  if constexpr (kUsesT<SCHEME> || VISC) r.T = P.T[m];
+  if constexpr (LOAD_S && kUsesSensor<SCHEME, ORDER>) r.s = P.s[ms];
+  if constexpr (VISC) {
+    r.mu = P.mu[m];
+    if constexpr (VISC == 2) {
+      r.mt = P.mut[m];
+      r.qc2 = P.qc2[m];
+    }
+    const S* f[4];
+    int st[4];
+    cross_fields<S, DIR>(P, g, f, st);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int t = 0; t < VO; ++t) {
+        const int off = t < VO / 2 ? t - VO / 2 : t - VO / 2 + 1;
+        r.x[q][t] = f[q][m + off * st[q]];
+      }
+    }
+  }
+}
+
+// cross_d over the raw stencil values (centre entry zero, never read)
 template <class R, class S, int VO>
-__device__ __forceinline__ R dcross(const PhysC<R>& c, const S* __restrict__ f, int m, int st, R inv) {
+__device__ __forceinline__ R cross_raw(const PhysC<R>& c, const S* x, R inv) {
   R a[VO + 1];
 #pragma unroll
-  for (int t = 0; t <= VO; ++t) a[t] = (t == VO / 2) ? R(0) : R(f[m + (t - VO / 2) * st]);
+  for (int t = 0; t <= VO; ++t) a[t] = (t == VO / 2) ? R(0) : R(x[t < VO / 2 ? t : t - 1]);
   return cross_d<R, VO>(c, inv, a);
+}
+
+// Turns the loaded values into the window cell (conversions, cross
+// derivatives, LES total enthalpy).
+template <class R, class S, int DIR, int ORDER, int SCHEME, int VISC, int VO, bool LOAD_S = true>
+__device__ __forceinline__ void finish_cell(const Geom& g, const PhysC<R>& c, const RawCell<S, VISC, VO>& r,
+                                            CellData<R, DIR, ORDER, SCHEME, VISC, VO>& d) {
+  d.rho = R(r.rho);
+  d.u = R(r.u);
+  d.v = R(r.v);
+  d.w = R(r.w);
+  d.p = R(r.p);
+  if constexpr (kUsesT<SCHEME> || VISC) d.T = R(r.T);
+  if constexpr (LOAD_S && kUsesSensor<SCHEME, ORDER>) d.s = R(r.s);
+  if constexpr (VISC) {
+    d.mu = R(r.mu);
+    if constexpr (VISC == 2) {
+      d.mt = R(r.mt);
+      d.H = c.Cp * d.T + R(0.5) * (d.u * d.u + d.v * d.v + d.w * d.w) + R(r.qc2);
+    }
+    const R ix = R(g.inv_dx), iy = R(g.inv_dy), iz = R(g.inv_dz);
+    const R i0 = DIR == 0 ? iy : ix, i1 = DIR == 0 ? iy : ix;
+    const R i2 = DIR == 2 ? iy : iz, i3 = DIR == 2 ? iy : iz;
+    d.d1 = cross_raw<R, S, VO>(c, r.x[0], i0);
+    d.d2 = cross_raw<R, S, VO>(c, r.x[1], i1);
+    d.d3 = cross_raw<R, S, VO>(c, r.x[2], i2);
+    d.d4 = cross_raw<R, S, VO>(c, r.x[3], i3);
+  }
 }
 
 // Loads the primitive values (and, for viscous runs, the cross derivatives a
@@ -232,37 +315,9 @@ __device__ __forceinline__ R dcross(const PhysC<R>& c, const S* __restrict__ f, 
 template <class R, class S, int DIR, int ORDER, int SCHEME, int VISC, int VO, bool LOAD_S = true>
 __device__ __forceinline__ void load_cell(const Prim<S>& P, const Geom& g, const PhysC<R>& c, int m, int ms,
                                           CellData<R, DIR, ORDER, SCHEME, VISC, VO>& d) {
-  d.rho = R(P.rho[m]);
-  d.u = R(P.u[m]);
-  d.v = R(P.v[m]);
-  d.w = R(P.w[m]);
-  d.p = R(P.p[m]);
-  if constexpr (kUsesT<SCHEME> || VISC) d.T = R(P.T[m]);
-  if constexpr (LOAD_S && kUsesSensor<SCHEME, ORDER>) d.s = R(P.s[ms]);
-  if constexpr (VISC) {
-    d.mu = R(P.mu[m]);
-    if constexpr (VISC == 2) {
-      d.mt = R(P.mut[m]);
-      d.H = c.Cp * d.T + R(0.5) * (d.u * d.u + d.v * d.v + d.w * d.w) + R(P.qc2[m]);
-    }
-    const int sx = 1, sy = g.nx, sz = g.nx * g.ny;
-    if constexpr (DIR == 0) {  // uy, vy, uz, wz
-      d.d1 = dcross<R, S, VO>(c, P.u, m, sy, R(g.inv_dy));
-      d.d2 = dcross<R, S, VO>(c, P.v, m, sy, R(g.inv_dy));
-      d.d3 = dcross<R, S, VO>(c, P.u, m, sz, R(g.inv_dz));
-      d.d4 = dcross<R, S, VO>(c, P.w, m, sz, R(g.inv_dz));
-    } else if constexpr (DIR == 1) {  // ux, vx, vz, wz
-      d.d1 = dcross<R, S, VO>(c, P.u, m, sx, R(g.inv_dx));
-      d.d2 = dcross<R, S, VO>(c, P.v, m, sx, R(g.inv_dx));
-      d.d3 = dcross<R, S, VO>(c, P.v, m, sz, R(g.inv_dz));
-      d.d4 = dcross<R, S, VO>(c, P.w, m, sz, R(g.inv_dz));
-    } else {  // ux, wx, vy, wy
-      d.d1 = dcross<R, S, VO>(c, P.u, m, sx, R(g.inv_dx));
-      d.d2 = dcross<R, S, VO>(c, P.w, m, sx, R(g.inv_dx));
-      d.d3 = dcross<R, S, VO>(c, P.v, m, sy, R(g.inv_dy));
-      d.d4 = dcross<R, S, VO>(c, P.w, m, sy, R(g.inv_dy));
-    }
-  }
+  RawCell<S, VISC, VO> r;
+  fetch_cell<S, DIR, ORDER, SCHEME, VISC, VO, LOAD_S>(P, g, m, ms, r);
+  finish_cell<R, S, DIR, ORDER, SCHEME, VISC, VO, LOAD_S>(g, c, r, d);
 }
 
 // Face flux (convective minus viscous) from SW window cells w[0..SW-1]; the
@@ -320,12 +375,13 @@ __device__ __forceinline__ void face_flux(const PhysC<R>& c, int tvd, int varian
 }
 
 // ---------------------------------------------------------------------------
-// x sweep: one wavefront per grid row; 64 faces -> 63 cells per tile; the
-// right-face flux of each cell comes from the neighbouring lane by a
-// cross-lane shuffle.  Writes Racc = c_x * (E_right - E_left) (the first term
-// of calc_R).
+// x sweep.  One wavefront per grid row: the tile of a row is exactly 64 cells
+// wide (XT = 64 - SW interior cells plus the stencil halo), so every lane
+// loads one cell, and lanes 0..XT compute the XT+1 faces the interior cells
+// need; the right-face flux of each cell comes from the neighbouring lane by
+// a cross-lane shuffle.
 // ---------------------------------------------------------------------------
-constexpr int XT = 63;  // cells per x tile
+template <int ORDER, int VISC, int VO> constexpr int xtile() { return 64 - Sten<ORDER, VISC, VO>::SW; }
 constexpr int XROWS = 4;
 
 // Ducros sensor of one cell (calc_Ducros, 3D_solver/src/calc_hybrid.f90:11-72)
@@ -353,20 +409,60 @@ __device__ __forceinline__ R ducros(const Geom& g, R uxm, R uxp, R vxm, R vxp, R
   return fmin2(R(1), fd);
 }
 
-// x sweep: one wavefront per grid row; 64 faces -> 63 cells per tile; the
-// right-face flux of each cell comes from the neighbouring lane by a
-// cross-lane shuffle.  Writes Racc = c_x * (E_right - E_left) (the first term
-// of calc_R).  For SLAU the Ducros sensor is computed here for the row (its x
-// neighbours are in the LDS row) and stored for the y and z sweeps.
-// The LDS tile is structure-of-arrays so that lane-consecutive reads are
-// bank-conflict free.
+// The same sensor in two parts: the y/z-derivative terms (from loads the
+// cell's own thread issues) and the completion with the x derivatives (from
+// the LDS row).  Same operations in the same order as ducros().
+template <class R> struct DucrosPart {
+  R dudy, dvdy, dwdz, dudz, r1;
+};
+template <class R>
+__device__ __forceinline__ DucrosPart<R> ducros_yz(const Geom& g, R uym, R uyp, R vym, R vyp, R wym, R wyp, R uzm,
+                                                   R uzp, R vzm, R vzp, R wzm, R wzp) {
+  const R dyt = R(0.25) * (R(g.inv_dy) + R(g.inv_dy));
+  const R dzt = R(0.25) * (R(g.inv_dz) + R(g.inv_dz));
+  DucrosPart<R> d;
+  d.dudy = (-uym + uyp) * dyt;
+  d.dvdy = (-vym + vyp) * dyt;
+  const R dwdy = (-wym + wyp) * dyt;
+  d.dudz = (-uzm + uzp) * dzt;
+  const R dvdz = (-vzm + vzp) * dzt;
+  d.dwdz = (-wzm + wzp) * dzt;
+  d.r1 = dwdy - dvdz;
+  return d;
+}
+template <class R>
+__device__ __forceinline__ R ducros_x(const Geom& g, const DucrosPart<R>& d, R uxm, R uxp, R vxm, R vxp, R wxm,
+                                      R wxp) {
+  const R dxt = R(0.25) * (R(g.inv_dx) + R(g.inv_dx));
+  const R dudx = (-uxm + uxp) * dxt;
+  const R dvdx = (-vxm + vxp) * dxt;
+  const R dwdx = (-wxm + wxp) * dxt;
+  const R div = dudx + d.dvdy + d.dwdz;
+  const R r2 = d.dudz - dwdx;
+  const R r3 = dvdx - d.dudy;
+  R fd = (div * div) / (div * div + (d.r1 * d.r1 + r2 * r2 + r3 * r3) + R(1.0e-12));
+  return fmin2(R(1), fd);
+}
+
+// Writes Racc = c_x * (E_right - E_left) (the first term of calc_R).  For
+// the schemes that read the Ducros sensor the sweep also computes it for its
+// row and stores it for the y and z sweeps.  The LDS tile is
+// structure-of-arrays so that lane-consecutive reads are bank-conflict free.
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
 __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__ Racc, Geom g, PhysC<R> c, int tvd,
                                                  int variant, double cx) {
   using St = Sten<ORDER, VISC, VO>;
   using CD = CellData<R, 0, ORDER, SCHEME, VISC, VO>;
+  using RC = RawCell<S, VISC, VO>;
   constexpr bool SENS = kUsesSensor<SCHEME, ORDER>;
+  constexpr int XT = xtile<ORDER, VISC, VO>();
   constexpr int W = XT + St::SW;
+  static_assert(W == 64, "one tile cell per lane");
+  // With one ghost layer (ORDER 2) a ghost cell's sensor is the boundary copy
+  // of its interior neighbour (calc_hybrid.f90:54-71), taken from a clamped
+  // index; with more layers every sensor the faces read belongs to a cell in
+  // 1..nx-2 whose x neighbours are in the row tile.
+  constexpr bool SPLIT_SENS = SENS && ORDER > 2;
   constexpr int NF = VISC == 2 ? 14 : 12;  // rho,u,v,w,p,T,mu,s,d1..d4[,mt,H]
   __shared__ R tile[NF][XROWS][W];
   const int tx = threadIdx.x, ty = threadIdx.y;
@@ -374,62 +470,90 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__
   const int j = g.h + blockIdx.y * XROWS + ty;
   const int k = g.h + blockIdx.z;
   const bool rowok = j < g.h + g.iy;
-  for (int m = tx; m < W; m += 64) {
-    const int i = i0 - St::HW + m;
-    if (rowok && i < g.nx) {
-      CD d;
-      load_cell<R, S, 0, ORDER, SCHEME, VISC, VO, false>(P, g, c, cidx(g, i, j, k), 0, d);
-      tile[0][ty][m] = d.rho; tile[1][ty][m] = d.u; tile[2][ty][m] = d.v; tile[3][ty][m] = d.w;
-      tile[4][ty][m] = d.p;
-      if constexpr (kUsesT<SCHEME> || VISC) tile[5][ty][m] = d.T;
-      if constexpr (VISC) {
-        tile[6][ty][m] = d.mu;
-        tile[8][ty][m] = d.d1; tile[9][ty][m] = d.d2; tile[10][ty][m] = d.d3; tile[11][ty][m] = d.d4;
-        if constexpr (VISC == 2) { tile[NF - 2][ty][m] = d.mt; tile[NF - 1][ty][m] = d.H; }
+  const int m = tx;                // this lane's tile position
+  const int i = i0 - St::HW + m;   // and its cell
+  const int sy = g.nx, sz = g.nx * g.ny;
+  // sensor positions: the cells either side of the XT+1 faces
+  const bool spos = m >= St::HW - 1 && m <= St::HW + XT && i <= g.nx - 2;
+  DucrosPart<R> dp{};
+  if (rowok && i < g.nx) {
+    const int q = cidx(g, i, j, k);
+    RC r;
+    fetch_cell<S, 0, ORDER, SCHEME, VISC, VO, false>(P, g, q, 0, r);
+    if constexpr (SPLIT_SENS) {
+      if (spos) {
+        constexpr int H = VO / 2;  // r.x[.][H-1], r.x[.][H]: the -1 and +1 neighbours
+        R uym, uyp, vym, vyp, uzm, uzp, wzm, wzp;
+        if constexpr (VISC) {  // DIR 0 cross stencils: u,v along y; u,w along z
+          uym = R(r.x[0][H - 1]); uyp = R(r.x[0][H]);
+          vym = R(r.x[1][H - 1]); vyp = R(r.x[1][H]);
+          uzm = R(r.x[2][H - 1]); uzp = R(r.x[2][H]);
+          wzm = R(r.x[3][H - 1]); wzp = R(r.x[3][H]);
+        } else {
+          uym = R(P.u[q - sy]); uyp = R(P.u[q + sy]);
+          vym = R(P.v[q - sy]); vyp = R(P.v[q + sy]);
+          uzm = R(P.u[q - sz]); uzp = R(P.u[q + sz]);
+          wzm = R(P.w[q - sz]); wzp = R(P.w[q + sz]);
+        }
+        dp = ducros_yz<R>(g, uym, uyp, vym, vyp, R(P.w[q - sy]), R(P.w[q + sy]), uzm, uzp, R(P.v[q - sz]),
+                          R(P.v[q + sz]), wzm, wzp);
       }
+    }
+    CD d;
+    finish_cell<R, S, 0, ORDER, SCHEME, VISC, VO, false>(g, c, r, d);
+    tile[0][ty][m] = d.rho; tile[1][ty][m] = d.u; tile[2][ty][m] = d.v; tile[3][ty][m] = d.w;
+    tile[4][ty][m] = d.p;
+    if constexpr (kUsesT<SCHEME> || VISC) tile[5][ty][m] = d.T;
+    if constexpr (VISC) {
+      tile[6][ty][m] = d.mu;
+      tile[8][ty][m] = d.d1; tile[9][ty][m] = d.d2; tile[10][ty][m] = d.d3; tile[11][ty][m] = d.d4;
+      if constexpr (VISC == 2) { tile[NF - 2][ty][m] = d.mt; tile[NF - 1][ty][m] = d.H; }
     }
   }
   __syncthreads();
   if constexpr (SENS) {
-    // sensor at tile positions HW-1 .. HW+63 (the cells either side of the 64
-    // faces).  With one ghost layer (ORDER 2) the ghost cell's sensor is the
-    // boundary copy of its interior neighbour (calc_hybrid.f90:54-71); with
-    // more layers the ghost cells used are computed like interior ones.
-    for (int m = St::HW - 1 + tx; m <= St::HW + 63; m += 64) {
-      const int i = i0 - St::HW + m;
-      if (rowok && i <= g.nx - 1) {
-        // clamping also keeps the unused positions past the last face in bounds
+    if constexpr (SPLIT_SENS) {
+      if (rowok && spos) {
+        const R fd = ducros_x<R>(g, dp, tile[1][ty][m - 1], tile[1][ty][m + 1], tile[2][ty][m - 1],
+                                 tile[2][ty][m + 1], tile[3][ty][m - 1], tile[3][ty][m + 1]);
+        tile[7][ty][m] = fd;
+        if (i >= i0 && i < i0 + XT && i < g.h + g.ix) P.s[cidx(g, i, j, k)] = (S)fd;  // one tile stores a cell
+      }
+    } else {
+      if (rowok && m >= St::HW - 1 && m <= St::HW + XT && i <= g.nx - 1) {
         const int ic = min(max(i, 1), g.nx - 2);
         const int q = cidx(g, ic, j, k);
-        const int sy = g.nx, sz = g.nx * g.ny;
         const R fd = ducros<R>(g, R(P.u[q - 1]), R(P.u[q + 1]), R(P.v[q - 1]), R(P.v[q + 1]), R(P.w[q - 1]),
                                R(P.w[q + 1]), R(P.u[q - sy]), R(P.u[q + sy]), R(P.v[q - sy]), R(P.v[q + sy]),
                                R(P.w[q - sy]), R(P.w[q + sy]), R(P.u[q - sz]), R(P.u[q + sz]), R(P.v[q - sz]),
                                R(P.v[q + sz]), R(P.w[q - sz]), R(P.w[q + sz]));
         tile[7][ty][m] = fd;
-        if (i >= i0 && i < i0 + XT && i < g.h + g.ix) P.s[q] = (S)fd;  // each cell stored by one tile
+        if (i >= i0 && i < i0 + XT && i < g.h + g.ix) P.s[q] = (S)fd;
       }
     }
     __syncthreads();
   }
   if (!rowok) return;
-  // face f = i0 - 1 + tx: window cells f-HW+1 .. f+HW = tile positions tx .. tx+SW-1
+  // face f = i0 - 1 + tx: window cells f-HW+1 .. f+HW = tile positions
+  // tx .. tx+SW-1 (< W for the XT+1 faces the tile's cells need)
   R F[5] = {R(0), R(0), R(0), R(0), R(0)};
   const int f = i0 - 1 + tx;
-  const bool face_ok = f <= g.h + g.ix - 1;
+  const bool face_ok = tx <= XT && f <= g.h + g.ix - 1;
   if (face_ok) {
     CD win[St::SW];
 #pragma unroll
-    for (int m = 0; m < St::SW; ++m) {
-      win[m].rho = tile[0][ty][tx + m]; win[m].u = tile[1][ty][tx + m]; win[m].v = tile[2][ty][tx + m];
-      win[m].w = tile[3][ty][tx + m]; win[m].p = tile[4][ty][tx + m];
-      if constexpr (kUsesT<SCHEME> || VISC) win[m].T = tile[5][ty][tx + m];
-      if constexpr (SENS) win[m].s = tile[7][ty][tx + m];
+    for (int n = 0; n < St::SW; ++n) {
+      win[n].rho = tile[0][ty][tx + n]; win[n].u = tile[1][ty][tx + n]; win[n].v = tile[2][ty][tx + n];
+      win[n].w = tile[3][ty][tx + n]; win[n].p = tile[4][ty][tx + n];
+      if constexpr (kUsesT<SCHEME> || VISC) win[n].T = tile[5][ty][tx + n];
+      if constexpr (SENS) {
+        if (n == St::HW - 1 || n == St::HW) win[n].s = tile[7][ty][tx + n];
+      }
       if constexpr (VISC) {
-        win[m].mu = tile[6][ty][tx + m];
-        win[m].d1 = tile[8][ty][tx + m]; win[m].d2 = tile[9][ty][tx + m];
-        win[m].d3 = tile[10][ty][tx + m]; win[m].d4 = tile[11][ty][tx + m];
-        if constexpr (VISC == 2) { win[m].mt = tile[NF - 2][ty][tx + m]; win[m].H = tile[NF - 1][ty][tx + m]; }
+        win[n].mu = tile[6][ty][tx + n];
+        win[n].d1 = tile[8][ty][tx + n]; win[n].d2 = tile[9][ty][tx + n];
+        win[n].d3 = tile[10][ty][tx + n]; win[n].d4 = tile[11][ty][tx + n];
+        if constexpr (VISC == 2) { win[n].mt = tile[NF - 2][ty][tx + n]; win[n].H = tile[NF - 1][ty][tx + n]; }
       }
     }
     face_flux<R, 0, ORDER, SCHEME, VISC, VO>(c, tvd, variant, R(g.inv_dx), win, F);
@@ -437,10 +561,10 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__
   double Fr[5];
 #pragma unroll
   for (int l = 0; l < 5; ++l) Fr[l] = __shfl_down((double)F[l], 1, 64);
-  const int i = i0 + tx;
-  if (tx < XT && i < g.h + g.ix) {
+  const int ic = i0 + tx;
+  if (tx < XT && ic < g.h + g.ix) {
 #pragma unroll
-    for (int l = 0; l < 5; ++l) Racc[qidx(g, i, l, j, k)] = cx * (-(double)F[l] + Fr[l]);
+    for (int l = 0; l < 5; ++l) Racc[qidx(g, ic, l, j, k)] = cx * (-(double)F[l] + Fr[l]);
   }
 }
 
@@ -451,7 +575,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__
 // R += c_y*(F_right - F_left); DIR=2 adds c_z*(G_right - G_left) and applies
 // the stage epilogue.
 // ---------------------------------------------------------------------------
-constexpr int LSEG = 32;
+constexpr int LSEG = 64;
 
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO, int DIR, int EP>
 __global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, double* __restrict__ Racc, Geom g, PhysC<R> c,
@@ -478,24 +602,55 @@ __global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, double* __restric
     if constexpr (ORDER == 2) return sp < g.h ? g.h : (sp >= g.h + nd ? g.h + nd - 1 : sp);
     return sp < g.h ? sp + nd : (sp >= g.h + nd ? sp - nd : sp);
   };
+  using RC = RawCell<S, VISC, VO>;
   CD win[St::SW];
 #pragma unroll
   for (int m = 0; m < St::SW - 1; ++m) {
     const int sp = s0 - St::HW + m;
     load_cell<R, S, DIR, ORDER, SCHEME, VISC, VO>(P, g, c, mbase + sp * stride, mbase + wrap(sp) * stride, win[m]);
   }
+  // Software pipeline: the values of the cell that enters the window at face
+  // f+1 are loaded while face f is computed, and the residual (and state) of
+  // cell f are loaded before its face flux, so that one face's arithmetic
+  // covers the memory latency of the next.
+  RC nxt;
+  {
+    const int sp = s0 - 1 + St::HW;
+    fetch_cell<S, DIR, ORDER, SCHEME, VISC, VO>(P, g, mbase + sp * stride, mbase + wrap(sp) * stride, nxt);
+  }
   R Fp[5];
   for (int f = s0 - 1; f < s_end; ++f) {
-    const int sp = f + St::HW;
-    load_cell<R, S, DIR, ORDER, SCHEME, VISC, VO>(P, g, c, mbase + sp * stride, mbase + wrap(sp) * stride,
-                                                  win[St::SW - 1]);
+    finish_cell<R, S, DIR, ORDER, SCHEME, VISC, VO>(g, c, nxt, win[St::SW - 1]);
+    if (f + 1 < s_end) {
+      const int sp = f + 1 + St::HW;
+      fetch_cell<S, DIR, ORDER, SCHEME, VISC, VO>(P, g, mbase + sp * stride, mbase + wrap(sp) * stride, nxt);
+    }
+    const int q = qbase + f * qstride;  // cell f (read only when f >= s0)
+    double Ra[5], Qa[5], Qb[5], Qc[5];  // residual; Qin, Qout, Rs of cell f
+    if (f >= s0) {
+#pragma unroll
+      for (int l = 0; l < 5; ++l) Ra[l] = Racc[q + l * g.nx];
+      if constexpr (DIR == 2) {
+        if constexpr (EP == EP_RK3_S1 || EP == EP_RK3_S23 || EP == EP_RK4_S) {
+#pragma unroll
+          for (int l = 0; l < 5; ++l) Qa[l] = sa.Qin[q + l * g.nx];
+        }
+        if constexpr (EP == EP_RK3_S23 || EP == EP_RK4_S4) {
+#pragma unroll
+          for (int l = 0; l < 5; ++l) Qb[l] = sa.Qout[q + l * g.nx];
+        }
+        if constexpr (EP == EP_RK4_S || EP == EP_RK4_S4) {
+#pragma unroll
+          for (int l = 0; l < 5; ++l) Qc[l] = sa.Rs[q + l * g.nx];
+        }
+      }
+    }
     R F[5];
     face_flux<R, DIR, ORDER, SCHEME, VISC, VO>(c, tvd, variant, inv_d, win, F);
     if (f >= s0) {
-      const int q = qbase + f * qstride;  // cell f
       double Rl[5];
 #pragma unroll
-      for (int l = 0; l < 5; ++l) Rl[l] = Racc[q + l * g.nx] + cd * (-(double)Fp[l] + (double)F[l]);
+      for (int l = 0; l < 5; ++l) Rl[l] = Ra[l] + cd * (-(double)Fp[l] + (double)F[l]);
       if constexpr (DIR == 1) {
 #pragma unroll
         for (int l = 0; l < 5; ++l) Racc[q + l * g.nx] = Rl[l];
@@ -503,26 +658,23 @@ __global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, double* __restric
         // stage epilogue (3D_solver/src/calc_steps.f90:60-69, :98-106, :137-145, :173-183)
         if constexpr (EP == EP_RK3_S1) {
 #pragma unroll
-          for (int l = 0; l < 5; ++l) sa.Qout[q + l * g.nx] = sa.Qin[q + l * g.nx] - Rl[l];
+          for (int l = 0; l < 5; ++l) sa.Qout[q + l * g.nx] = Qa[l] - Rl[l];
         } else if constexpr (EP == EP_RK3_S23) {
 #pragma unroll
-          for (int l = 0; l < 5; ++l) {
-            const int a = q + l * g.nx;
-            sa.Qout[a] = (sa.c1 * sa.Qin[a] + sa.c2 * sa.Qout[a] - Rl[l]) * sa.c4inv;
-          }
+          for (int l = 0; l < 5; ++l) sa.Qout[q + l * g.nx] = (sa.c1 * Qa[l] + sa.c2 * Qb[l] - Rl[l]) * sa.c4inv;
         } else if constexpr (EP == EP_RK4_S) {
 #pragma unroll
           for (int l = 0; l < 5; ++l) {
             const int a = q + l * g.nx;
-            sa.Qout[a] = sa.Qin[a] - sa.c1 * Rl[l];
-            sa.Rs[a] = sa.Rs[a] + sa.c2 * Rl[l];
+            sa.Qout[a] = Qa[l] - sa.c1 * Rl[l];
+            sa.Rs[a] = Qc[l] + sa.c2 * Rl[l];
           }
         } else if constexpr (EP == EP_RK4_S4) {
 #pragma unroll
           for (int l = 0; l < 5; ++l) {
             const int a = q + l * g.nx;
-            const double r = sa.Rs[a] + Rl[l];
-            sa.Qout[a] = sa.Qout[a] - r * sa.c1;  // c1 = one_sixth
+            const double r = Qc[l] + Rl[l];
+            sa.Qout[a] = Qb[l] - r * sa.c1;  // c1 = one_sixth
             sa.Rs[a] = 0.0;
           }
         } else {

@@ -161,13 +161,22 @@ template <class R> OX_HD R d33(R d1, R d2, R d3) {
          minmod3(d3, R(2) * d1, R(2) * d2);
 }
 
-// delta4 -> MUSCL3rd{nonTVD,Minmod,Threshold} (:44-91, :142-153)
-template <class R>
-OX_HD void delta4(const PhysC<R>& c, int tvd, R sensor, const R* a, R& al, R& ar) {
+// True where the MUSCL interface states take the limited form: TVD kind
+// "tvd" everywhere, kind "hybrid" where the face sensor reaches the threshold
+// (the MUSCL*Threshold selectors of src/calc_muscl.f90:142-169).  The switch
+// is taken once per face, so a wavefront whose faces are all smooth skips the
+// limiter arithmetic entirely.
+template <class R> OX_HD bool muscl_limited(const PhysC<R>& c, int tvd, R sensor) {
+  return !(tvd == 0 || (tvd == 2 && sensor < c.threshold));
+}
+
+// delta4 -> MUSCL3rd{nonTVD,Minmod} (:44-91)
+template <class R, bool LIM>
+OX_HD void delta4_form(const PhysC<R>& c, const R* a, R& al, R& ar) {
   const R d1 = -a[0] + a[1];
   const R d2 = -a[1] + a[2];
   const R d3 = -a[2] + a[3];
-  if (tvd == 0 || (tvd == 2 && sensor < c.threshold)) {
+  if constexpr (!LIM) {
     al = a[1] + R(0.5) * (d1 + R(2) * d2) * c.one_third;
     ar = a[2] - R(0.5) * (d3 + R(2) * d2) * c.one_third;
   } else {
@@ -181,15 +190,15 @@ OX_HD void delta4(const PhysC<R>& c, int tvd, R sensor, const R* a, R& al, R& ar
   }
 }
 
-// delta6 -> MUSCL4th{nonTVD,TVD,Threshold} (:94-139, :156-169)
-template <class R>
-OX_HD void delta6(const PhysC<R>& c, int tvd, R sensor, const R* a, R& al, R& ar) {
+// delta6 -> MUSCL4th{nonTVD,TVD} (:94-139)
+template <class R, bool LIM>
+OX_HD void delta6_form(const PhysC<R>& c, const R* a, R& al, R& ar) {
   const R d1 = -a[0] + a[1];
   const R d2 = -a[1] + a[2];
   const R d3 = -a[2] + a[3];
   const R d4 = -a[3] + a[4];
   const R d5 = -a[4] + a[5];
-  if (tvd == 0 || (tvd == 2 && sensor < c.threshold)) {
+  if constexpr (!LIM) {
     al = a[2] + (R(-0.4) * d1 + R(2.2) * d2 + R(4.8) * d3 - R(0.6) * d4) * c.one_twelfth;
     ar = a[3] - (R(-0.6) * d2 + R(4.8) * d3 + R(2.2) * d4 - R(0.4) * d5) * c.one_twelfth;
   } else {
@@ -206,6 +215,37 @@ OX_HD void delta6(const PhysC<R>& c, int tvd, R sensor, const R* a, R& al, R& ar
       const R dr = minmod2(delta3, R(4) * delta2);
       ar = a[3] - (dr + R(2) * dl) * c.one_sixth;
     }
+  }
+}
+
+// delta4 / delta6 with the selector (MUSCL3rdThreshold / MUSCL4thThreshold,
+// :142-169) for one variable.
+template <class R>
+OX_HD void delta4(const PhysC<R>& c, int tvd, R sensor, const R* a, R& al, R& ar) {
+  if (muscl_limited(c, tvd, sensor)) delta4_form<R, true>(c, a, al, ar);
+  else delta4_form<R, false>(c, a, al, ar);
+}
+template <class R>
+OX_HD void delta6(const PhysC<R>& c, int tvd, R sensor, const R* a, R& al, R& ar) {
+  if (muscl_limited(c, tvd, sensor)) delta6_form<R, true>(c, a, al, ar);
+  else delta6_form<R, false>(c, a, al, ar);
+}
+
+// The five primitives of one face in one form.
+template <class R, int N, bool LIM>
+OX_HD void muscl5(const PhysC<R>& c, const R* rho, const R* u, const R* v, const R* w, const R* p, R* L, R* Rr) {
+  if constexpr (N == 4) {
+    delta4_form<R, LIM>(c, rho, L[0], Rr[0]);
+    delta4_form<R, LIM>(c, u, L[1], Rr[1]);
+    delta4_form<R, LIM>(c, v, L[2], Rr[2]);
+    delta4_form<R, LIM>(c, w, L[3], Rr[3]);
+    delta4_form<R, LIM>(c, p, L[4], Rr[4]);
+  } else {
+    delta6_form<R, LIM>(c, rho, L[0], Rr[0]);
+    delta6_form<R, LIM>(c, u, L[1], Rr[1]);
+    delta6_form<R, LIM>(c, v, L[2], Rr[2]);
+    delta6_form<R, LIM>(c, w, L[3], Rr[3]);
+    delta6_form<R, LIM>(c, p, L[4], Rr[4]);
   }
 }
 
@@ -330,18 +370,9 @@ OX_HD void reconstruct(const PhysC<R>& c, int tvd, R fsens, const R* rho, const 
   if constexpr (N == 2) {
     L[0] = rho[0]; Rr[0] = rho[1]; L[1] = u[0]; Rr[1] = u[1]; L[2] = v[0]; Rr[2] = v[1];
     L[3] = w[0]; Rr[3] = w[1]; L[4] = p[0]; Rr[4] = p[1];
-  } else if constexpr (N == 4) {
-    delta4(c, tvd, fsens, rho, L[0], Rr[0]);
-    delta4(c, tvd, fsens, u, L[1], Rr[1]);
-    delta4(c, tvd, fsens, v, L[2], Rr[2]);
-    delta4(c, tvd, fsens, w, L[3], Rr[3]);
-    delta4(c, tvd, fsens, p, L[4], Rr[4]);
   } else {
-    delta6(c, tvd, fsens, rho, L[0], Rr[0]);
-    delta6(c, tvd, fsens, u, L[1], Rr[1]);
-    delta6(c, tvd, fsens, v, L[2], Rr[2]);
-    delta6(c, tvd, fsens, w, L[3], Rr[3]);
-    delta6(c, tvd, fsens, p, L[4], Rr[4]);
+    if (muscl_limited(c, tvd, fsens)) muscl5<R, N, true>(c, rho, u, v, w, p, L, Rr);
+    else muscl5<R, N, false>(c, rho, u, v, w, p, L, Rr);
   }
 }
 
@@ -366,26 +397,13 @@ OX_HD void roe_face(const PhysC<R>& c, int tvd, const R* rho, const R* u, const 
 template <class R, int DIR, int N>
 OX_HD void slau_face(const PhysC<R>& c, int tvd, int variant, const R* rho, const R* u,
                                           const R* v, const R* w, const R* p, R fsens, R* F) {
-  R rl, rr, ul, ur, vl, vr, wl, wr, pl, pr, hr;
-  if constexpr (N == 2) {
-    rl = rho[0]; rr = rho[1]; ul = u[0]; ur = u[1]; vl = v[0]; vr = v[1]; wl = w[0]; wr = w[1]; pl = p[0]; pr = p[1];
-    hr = R(1);
-  } else if constexpr (N == 4) {
-    delta4(c, tvd, fsens, rho, rl, rr);
-    delta4(c, tvd, fsens, u, ul, ur);
-    delta4(c, tvd, fsens, v, vl, vr);
-    delta4(c, tvd, fsens, w, wl, wr);
-    delta4(c, tvd, fsens, p, pl, pr);
-    hr = wiggle(p);
-  } else {
-    delta6(c, tvd, fsens, rho, rl, rr);
-    delta6(c, tvd, fsens, u, ul, ur);
-    delta6(c, tvd, fsens, v, vl, vr);
-    delta6(c, tvd, fsens, w, wl, wr);
-    delta6(c, tvd, fsens, p, pl, pr);
-    hr = wiggle(p + 1);
-  }
-  slau<R, DIR>(c, variant, rl, rr, ul, ur, vl, vr, wl, wr, pl, pr, hr, F);
+  R L[5], Rr[5];
+  reconstruct<R, N>(c, tvd, fsens, rho, u, v, w, p, L, Rr);
+  R hr;
+  if constexpr (N == 2) hr = R(1);
+  else if constexpr (N == 4) hr = wiggle(p);
+  else hr = wiggle(p + 1);
+  slau<R, DIR>(c, variant, L[0], Rr[0], L[1], Rr[1], L[2], Rr[2], L[3], Rr[3], L[4], Rr[4], hr, F);
 }
 
 // ---------------------------------------------------------------------------

@@ -41,7 +41,7 @@ void run_stage(Core& C, const double* Qsrc, const StageArgs& sa, double cx, doub
     prof_mark(C, "les_mut");
   }
   k_sweep_x<R, S, ORDER, SCHEME, VISC, VO>
-      <<<dim3(cdiv(g.ix, XT), cdiv(g.iy, XROWS), g.iz), blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd, C.variant, cx);
+      <<<dim3(cdiv(g.ix, xtile<ORDER, VISC, VO>()), cdiv(g.iy, XROWS), g.iz), blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd, C.variant, cx);
   prof_mark(C, "sweep_x");
   k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 1, EP_RHS>
       <<<dim3(cdiv(g.ix, 64), cdiv(g.iz, 4), cdiv(g.iy, LSEG)), blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd,

@@ -173,7 +173,8 @@ void destroy(ox_solver* s) {
   hipFree(s->Rs);
   hipFree(s->scratch);
   hipFree(s->C.Racc);
-  for (int m = 0; m < 10; ++m) hipFree(s->C.prim[m]);
+  for (int set = 0; set < 2; ++set)
+    for (int m = 0; m < 10; ++m) hipFree(s->C.prim[set][m]);
   for (hipEvent_t e : s->prof.pool) hipEventDestroy(e);
   if (s->ev0) hipEventDestroy(s->ev0);
   if (s->ev1) hipEventDestroy(s->ev1);
@@ -239,7 +240,10 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   C.tvd = cfg->tvd;
   C.variant = cfg->slau_variant;
   C.Racc = nullptr;
-  for (int m = 0; m < 10; ++m) C.prim[m] = nullptr;
+  for (int set = 0; set < 2; ++set)
+    for (int m = 0; m < 10; ++m) C.prim[set][m] = nullptr;
+  C.cur = 0;
+  C.prim_for = nullptr;
   C.stream = nullptr;
   C.prof = false;
   C.mark = &prof_record;
@@ -268,10 +272,11 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   bool ok = hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking) == hipSuccess;
   ok = ok && hipMalloc(&s->QJ, ns * sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc(&s->QJ2, ns * sizeof(double)) == hipSuccess;
-  ok = ok && hipMalloc(&C.Racc, ns * sizeof(double)) == hipSuccess;
+  ok = ok && hipMalloc(&C.Racc, ns * pe) == hipSuccess;
   if (cfg->rk == 4) ok = ok && hipMalloc(&s->Rs, ns * sizeof(double)) == hipSuccess;
   const int nprim = cfg->visc == OX_VISC_LES ? 10 : 8;
-  for (int m = 0; m < nprim && ok; ++m) ok = hipMalloc(&C.prim[m], nc * pe) == hipSuccess;
+  for (int set = 0; set < 2; ++set)
+    for (int m = 0; m < nprim && ok; ++m) ok = hipMalloc(&C.prim[set][m], nc * pe) == hipSuccess;
   s->scratch_bytes = 4096 * (2 * ox::DIAG_NQ + 1) * sizeof(double);
   ok = ok && hipMalloc(&s->scratch, s->scratch_bytes) == hipSuccess;
   ok = ok && hipEventCreate(&s->ev0) == hipSuccess && hipEventCreate(&s->ev1) == hipSuccess;
@@ -282,9 +287,10 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   // Zero everything once: ghost layers of QJ2 and the accumulators start defined.
   ok = hipMemsetAsync(s->QJ, 0, ns * sizeof(double), C.stream) == hipSuccess;
   ok = ok && hipMemsetAsync(s->QJ2, 0, ns * sizeof(double), C.stream) == hipSuccess;
-  ok = ok && hipMemsetAsync(C.Racc, 0, ns * sizeof(double), C.stream) == hipSuccess;
+  ok = ok && hipMemsetAsync(C.Racc, 0, ns * pe, C.stream) == hipSuccess;
   if (s->Rs) ok = ok && hipMemsetAsync(s->Rs, 0, ns * sizeof(double), C.stream) == hipSuccess;
-  for (int m = 0; m < nprim && ok; ++m) ok = hipMemsetAsync(C.prim[m], 0, nc * pe, C.stream) == hipSuccess;
+  for (int set = 0; set < 2; ++set)
+    for (int m = 0; m < nprim && ok; ++m) ok = hipMemsetAsync(C.prim[set][m], 0, nc * pe, C.stream) == hipSuccess;
   ok = ok && hipStreamSynchronize(C.stream) == hipSuccess;
   if (!ok) {
     destroy(s);
@@ -307,6 +313,7 @@ int ox_set_qj(ox_solver* s, const double* QJ) {
   const size_t n = state_elems(s->C.g);
   OX_HIP(hipMemcpyAsync(s->QJ, QJ, n * sizeof(double), hipMemcpyHostToDevice, s->C.stream));
   if (s->Rs) OX_HIP(hipMemsetAsync(s->Rs, 0, n * sizeof(double), s->C.stream));
+  s->C.prim_for = nullptr;  // the primitives no longer describe the state
   OX_HIP(hipStreamSynchronize(s->C.stream));
   return OX_OK;
 }
@@ -354,6 +361,13 @@ int ox_step(ox_solver* s, int32_t nsteps) {
     prof_record(&s->C, "start");
   }
   for (int t = 0; t < nsteps; ++t) run_rk_step(s);
+  if (nsteps > 0) {
+    // the stages keep the primitives of ghost cells current themselves; the
+    // state's own ghost layers are filled once, for its readers (get_state,
+    // diagnostics, the RHS hook)
+    ox::launch_ghost_fill(s->C, s->QJ);
+    ox::prof_mark(s->C, "ghost_fill");
+  }
   OX_HIP(hipGetLastError());
   OX_HIP(hipEventRecord(s->ev1, s->C.stream));
   OX_HIP(hipEventSynchronize(s->ev1));
@@ -388,6 +402,7 @@ int ox_rhs(ox_solver* s, double* R) {
   // the result goes to the second state buffer, which is then restored from
   // Q^n semantics: QJ2 is scratch between steps.
   OX_HIP(hipMemsetAsync(s->QJ2, 0, n * sizeof(double), s->C.stream));
+  if (s->C.prim_for == s->QJ2) s->C.prim_for = nullptr;
   ox::StageArgs sa{s->QJ, s->QJ2, s->Rs, 0, 0, 0, ox::EP_RHS};
   s->stage(s->C, s->QJ, sa, g.a_yz, g.a_zx, g.a_xy);
   OX_HIP(hipGetLastError());

@@ -58,6 +58,35 @@ __device__ __forceinline__ int cidx(const Geom& g, int i, int j, int k) { return
 // Primitive recovery.  calc_quantities_T_3D / calc_quantities_3D
 // (src/calc_physical_quantities.f90:60-116), always in FP64.
 // ---------------------------------------------------------------------------
+// Primitives of one cell from its conserved values (J * Q = physical).
+struct PrimVals {
+  double rho, u, v, w, p, T, mu;
+};
+template <bool VISC>
+__device__ __forceinline__ PrimVals prim_of(double q1, double q2, double q3, double q4, double q5, double J,
+                                            const PhysC<double>& c) {
+  PrimVals r;
+  const double over_Q1 = 1.0 / q1;
+  r.rho = J * q1;
+  r.u = q2 * over_Q1;
+  r.v = q3 * over_Q1;
+  r.w = q4 * over_Q1;
+  r.p = c.gamma_1 * (J * q5 - 0.5 * r.rho * (r.u * r.u + r.v * r.v + r.w * r.w));
+  r.T = r.p / (c.Rgas * r.rho);
+  if constexpr (VISC) r.mu = c.mu0c / (r.T + 111.0) * (r.T * fsqrt(r.T));
+  return r;
+}
+template <class S, bool VISC>
+__device__ __forceinline__ void store_prim(const Prim<S>& P, int m, const PrimVals& r) {
+  P.rho[m] = (S)r.rho;
+  P.u[m] = (S)r.u;
+  P.v[m] = (S)r.v;
+  P.w[m] = (S)r.w;
+  P.p[m] = (S)r.p;
+  P.T[m] = (S)r.T;
+  if constexpr (VISC) P.mu[m] = (S)r.mu;
+}
+
 template <class S, bool VISC>
 __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ QJ, Prim<S> P, Geom g, PhysC<double> c) {
   const int i = blockIdx.x * 64 + threadIdx.x;
@@ -66,23 +95,8 @@ __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ QJ, Pri
   if (i >= g.nx || j >= g.ny) return;
   const int q = qidx(g, i, 0, j, k);
   const int st = g.nx;
-  const double J = g.jac;
-  const double q1 = QJ[q], q2 = QJ[q + st], q3 = QJ[q + 2 * st], q4 = QJ[q + 3 * st], q5 = QJ[q + 4 * st];
-  const double over_Q1 = 1.0 / q1;
-  const double rho = J * q1;
-  const double u = q2 * over_Q1;
-  const double v = q3 * over_Q1;
-  const double w = q4 * over_Q1;
-  const double p = c.gamma_1 * (J * q5 - 0.5 * rho * (u * u + v * v + w * w));
-  const double T = p / (c.Rgas * rho);
-  const int m = cidx(g, i, j, k);
-  P.rho[m] = (S)rho;
-  P.u[m] = (S)u;
-  P.v[m] = (S)v;
-  P.w[m] = (S)w;
-  P.p[m] = (S)p;
-  P.T[m] = (S)T;
-  if constexpr (VISC) P.mu[m] = (S)(c.mu0c / (T + 111.0) * (T * fsqrt(T)));
+  const PrimVals r = prim_of<VISC>(QJ[q], QJ[q + st], QJ[q + 2 * st], QJ[q + 3 * st], QJ[q + 4 * st], g.jac, c);
+  store_prim<S, VISC>(P, cidx(g, i, j, k), r);
 }
 
 // ---------------------------------------------------------------------------
@@ -449,7 +463,7 @@ __device__ __forceinline__ R ducros_x(const Geom& g, const DucrosPart<R>& d, R u
 // row and stores it for the y and z sweeps.  The LDS tile is
 // structure-of-arrays so that lane-consecutive reads are bank-conflict free.
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
-__global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__ Racc, Geom g, PhysC<R> c, int tvd,
+__global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, S* __restrict__ Racc, Geom g, PhysC<R> c, int tvd,
                                                  int variant, double cx) {
   using St = Sten<ORDER, VISC, VO>;
   using CD = CellData<R, 0, ORDER, SCHEME, VISC, VO>;
@@ -564,7 +578,33 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__
   const int ic = i0 + tx;
   if (tx < XT && ic < g.h + g.ix) {
 #pragma unroll
-    for (int l = 0; l < 5; ++l) Racc[qidx(g, ic, l, j, k)] = cx * (-(double)F[l] + Fr[l]);
+    for (int l = 0; l < 5; ++l) Racc[qidx(g, ic, l, j, k)] = (S)(cx * (-(double)F[l] + Fr[l]));
+  }
+}
+
+// The residual of one cell and, for the z sweep, the state values its RK
+// epilogue reads (A = Qin, B = Qout, C = Rs), per conserved variable.
+template <int DIR, int EP> struct CellState {
+  double R[5], A[5], B[5], C[5];
+};
+template <int DIR, int EP, class S>
+__device__ __forceinline__ void fetch_state(const S* __restrict__ Racc, const StageArgs& sa, int q, int st,
+                                            CellState<DIR, EP>& cs) {
+#pragma unroll
+  for (int l = 0; l < 5; ++l) cs.R[l] = (double)Racc[q + l * st];
+  if constexpr (DIR == 2) {
+    if constexpr (EP == EP_RK3_S1 || EP == EP_RK3_S23 || EP == EP_RK4_S) {
+#pragma unroll
+      for (int l = 0; l < 5; ++l) cs.A[l] = sa.Qin[q + l * st];
+    }
+    if constexpr (EP == EP_RK3_S23 || EP == EP_RK4_S4) {
+#pragma unroll
+      for (int l = 0; l < 5; ++l) cs.B[l] = sa.Qout[q + l * st];
+    }
+    if constexpr (EP == EP_RK4_S || EP == EP_RK4_S4) {
+#pragma unroll
+      for (int l = 0; l < 5; ++l) cs.C[l] = sa.Rs[q + l * st];
+    }
   }
 }
 
@@ -578,8 +618,9 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, double* __restrict__
 constexpr int LSEG = 64;
 
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO, int DIR, int EP>
-__global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, double* __restrict__ Racc, Geom g, PhysC<R> c,
-                                                    int tvd, int variant, double cd, StageArgs sa) {
+__global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, S* __restrict__ Racc, Geom g, PhysC<R> c,
+                                                    int tvd, int variant, double cd, StageArgs sa, Prim<S> Pn,
+                                                    PhysC<double> c64) {
   using St = Sten<ORDER, VISC, VO>;
   using CD = CellData<R, DIR, ORDER, SCHEME, VISC, VO>;
   const int i = g.h + blockIdx.x * 64 + threadIdx.x;
@@ -610,8 +651,8 @@ __global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, double* __restric
     load_cell<R, S, DIR, ORDER, SCHEME, VISC, VO>(P, g, c, mbase + sp * stride, mbase + wrap(sp) * stride, win[m]);
   }
   // Software pipeline: the values of the cell that enters the window at face
-  // f+1 are loaded while face f is computed, and the residual (and state) of
-  // cell f are loaded before its face flux, so that one face's arithmetic
+  // f+1, and the residual (and state) of the cell that face f+1 completes,
+  // are loaded while face f is computed, so that one face's arithmetic
   // covers the memory latency of the next.
   RC nxt;
   {
@@ -619,32 +660,18 @@ __global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, double* __restric
     fetch_cell<S, DIR, ORDER, SCHEME, VISC, VO>(P, g, mbase + sp * stride, mbase + wrap(sp) * stride, nxt);
   }
   R Fp[5];
+  CellState<DIR, EP> cs;
   for (int f = s0 - 1; f < s_end; ++f) {
     finish_cell<R, S, DIR, ORDER, SCHEME, VISC, VO>(g, c, nxt, win[St::SW - 1]);
     if (f + 1 < s_end) {
       const int sp = f + 1 + St::HW;
       fetch_cell<S, DIR, ORDER, SCHEME, VISC, VO>(P, g, mbase + sp * stride, mbase + wrap(sp) * stride, nxt);
     }
-    const int q = qbase + f * qstride;  // cell f (read only when f >= s0)
-    double Ra[5], Qa[5], Qb[5], Qc[5];  // residual; Qin, Qout, Rs of cell f
-    if (f >= s0) {
-#pragma unroll
-      for (int l = 0; l < 5; ++l) Ra[l] = Racc[q + l * g.nx];
-      if constexpr (DIR == 2) {
-        if constexpr (EP == EP_RK3_S1 || EP == EP_RK3_S23 || EP == EP_RK4_S) {
-#pragma unroll
-          for (int l = 0; l < 5; ++l) Qa[l] = sa.Qin[q + l * g.nx];
-        }
-        if constexpr (EP == EP_RK3_S23 || EP == EP_RK4_S4) {
-#pragma unroll
-          for (int l = 0; l < 5; ++l) Qb[l] = sa.Qout[q + l * g.nx];
-        }
-        if constexpr (EP == EP_RK4_S || EP == EP_RK4_S4) {
-#pragma unroll
-          for (int l = 0; l < 5; ++l) Qc[l] = sa.Rs[q + l * g.nx];
-        }
-      }
-    }
+    // residual (and state) of cell f+1, consumed one iteration later
+    CellState<DIR, EP> ncs;
+    if (f + 1 < s_end) fetch_state<DIR, EP>(Racc, sa, qbase + (f + 1) * qstride, g.nx, ncs);
+    const int q = qbase + f * qstride;  // cell f
+    const double *Ra = cs.R, *Qa = cs.A, *Qb = cs.B, *Qc = cs.C;
     R F[5];
     face_flux<R, DIR, ORDER, SCHEME, VISC, VO>(c, tvd, variant, inv_d, win, F);
     if (f >= s0) {
@@ -653,33 +680,53 @@ __global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, double* __restric
       for (int l = 0; l < 5; ++l) Rl[l] = Ra[l] + cd * (-(double)Fp[l] + (double)F[l]);
       if constexpr (DIR == 1) {
 #pragma unroll
-        for (int l = 0; l < 5; ++l) Racc[q + l * g.nx] = Rl[l];
+        for (int l = 0; l < 5; ++l) Racc[q + l * g.nx] = (S)Rl[l];
       } else {
         // stage epilogue (3D_solver/src/calc_steps.f90:60-69, :98-106, :137-145, :173-183)
+        double Qn[5];
         if constexpr (EP == EP_RK3_S1) {
 #pragma unroll
-          for (int l = 0; l < 5; ++l) sa.Qout[q + l * g.nx] = Qa[l] - Rl[l];
+          for (int l = 0; l < 5; ++l) Qn[l] = Qa[l] - Rl[l];
         } else if constexpr (EP == EP_RK3_S23) {
 #pragma unroll
-          for (int l = 0; l < 5; ++l) sa.Qout[q + l * g.nx] = (sa.c1 * Qa[l] + sa.c2 * Qb[l] - Rl[l]) * sa.c4inv;
+          for (int l = 0; l < 5; ++l) Qn[l] = (sa.c1 * Qa[l] + sa.c2 * Qb[l] - Rl[l]) * sa.c4inv;
         } else if constexpr (EP == EP_RK4_S) {
 #pragma unroll
           for (int l = 0; l < 5; ++l) {
-            const int a = q + l * g.nx;
-            sa.Qout[a] = Qa[l] - sa.c1 * Rl[l];
-            sa.Rs[a] = Qc[l] + sa.c2 * Rl[l];
+            Qn[l] = Qa[l] - sa.c1 * Rl[l];
+            sa.Rs[q + l * g.nx] = Qc[l] + sa.c2 * Rl[l];
           }
         } else if constexpr (EP == EP_RK4_S4) {
 #pragma unroll
           for (int l = 0; l < 5; ++l) {
-            const int a = q + l * g.nx;
             const double r = Qc[l] + Rl[l];
-            sa.Qout[a] = Qb[l] - r * sa.c1;  // c1 = one_sixth
-            sa.Rs[a] = 0.0;
+            Qn[l] = Qb[l] - r * sa.c1;  // c1 = one_sixth
+            sa.Rs[q + l * g.nx] = 0.0;
           }
         } else {
 #pragma unroll
-          for (int l = 0; l < 5; ++l) sa.Qout[q + l * g.nx] = Rl[l];
+          for (int l = 0; l < 5; ++l) Qn[l] = Rl[l];
+        }
+#pragma unroll
+        for (int l = 0; l < 5; ++l) sa.Qout[q + l * g.nx] = Qn[l];
+        if constexpr (EP != EP_RHS) {
+          // primitives of the new state for the next stage (what k_prim
+          // would compute from it), into the other primitive set, at the
+          // cell and at every ghost cell that is its periodic image (the
+          // ghost fill of set_bc_cyclic applied to the primitives)
+          const PrimVals pv = prim_of<(VISC != 0)>(Qn[0], Qn[1], Qn[2], Qn[3], Qn[4], g.jac, c64);
+          const int jj = t, kk = f;
+          store_prim<S, (VISC != 0)>(Pn, cidx(g, i, jj, kk), pv);
+          const int ai = i < 2 * g.h ? i + g.ix : (i >= g.ix ? i - g.ix : -1);
+          const int aj = jj < 2 * g.h ? jj + g.iy : (jj >= g.iy ? jj - g.iy : -1);
+          const int ak = kk < 2 * g.h ? kk + g.iz : (kk >= g.iz ? kk - g.iz : -1);
+          if (ai >= 0 || aj >= 0 || ak >= 0) {
+            const int xs[2] = {i, ai}, ys[2] = {jj, aj}, zs[2] = {kk, ak};
+            for (int c3 = 1; c3 < 8; ++c3) {
+              const int x = xs[c3 & 1], y = ys[(c3 >> 1) & 1], z = zs[(c3 >> 2) & 1];
+              if (x >= 0 && y >= 0 && z >= 0) store_prim<S, (VISC != 0)>(Pn, cidx(g, x, y, z), pv);
+            }
+          }
         }
       }
     }
@@ -687,6 +734,7 @@ __global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, double* __restric
     for (int l = 0; l < 5; ++l) Fp[l] = F[l];
 #pragma unroll
     for (int m = 0; m < St::SW - 1; ++m) win[m] = win[m + 1];
+    cs = ncs;
   }
 }
 

@@ -14,25 +14,26 @@ namespace ox {
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
+template <class S> Prim<S> prim_set(Core& C, int set) {
+  Prim<S> P;
+  S** a[10] = {&P.rho, &P.u, &P.v, &P.w, &P.p, &P.T, &P.mu, &P.s, &P.mut, &P.qc2};
+  for (int m = 0; m < 10; ++m) *a[m] = (S*)C.prim[set][m];
+  return P;
+}
+
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
 void run_stage(Core& C, const double* Qsrc, const StageArgs& sa, double cx, double cy, double cz) {
   const Geom& g = C.g;
-  Prim<S> P;
-  P.rho = (S*)C.prim[0];
-  P.u = (S*)C.prim[1];
-  P.v = (S*)C.prim[2];
-  P.w = (S*)C.prim[3];
-  P.p = (S*)C.prim[4];
-  P.T = (S*)C.prim[5];
-  P.mu = (S*)C.prim[6];
-  P.s = (S*)C.prim[7];
-  P.mut = (S*)C.prim[8];
-  P.qc2 = (S*)C.prim[9];
+  const Prim<S> P = prim_set<S>(C, C.cur);
+  const Prim<S> Pn = prim_set<S>(C, 1 - C.cur);
   const PhysC<double> c64 = make_physc<double>(C.hc);
   const PhysC<R> cr = make_physc<R>(C.hc);
   const dim3 blk(64, 4, 1);
-  k_prim<S, (VISC != 0)><<<dim3(cdiv(g.nx, 64), cdiv(g.ny, 4), g.nz), blk, 0, C.stream>>>(Qsrc, P, g, c64);
-  prof_mark(C, "prim");
+  if (C.prim_for != Qsrc) {
+    k_prim<S, (VISC != 0)><<<dim3(cdiv(g.nx, 64), cdiv(g.ny, 4), g.nz), blk, 0, C.stream>>>(Qsrc, P, g, c64);
+    C.prim_for = Qsrc;
+    prof_mark(C, "prim");
+  }
   if constexpr (VISC == 2) {
     k_les_mut<S, R><<<dim3(cdiv(g.nx - 2, 64), cdiv(g.ny - 2, 4), g.nz - 2), blk, 0, C.stream>>>(P, g, cr);
     const long long n0 = (long long)(g.ny - 2) * (g.nz - 2) + (long long)(g.nx - 2) * (g.nz - 2) + (g.nz - 2);
@@ -40,17 +41,17 @@ void run_stage(Core& C, const double* Qsrc, const StageArgs& sa, double cx, doub
     k_les_bc<S><<<(unsigned)(((long long)g.nx * g.ny + 255) / 256), 256, 0, C.stream>>>(P, g, 1);
     prof_mark(C, "les_mut");
   }
-  k_sweep_x<R, S, ORDER, SCHEME, VISC, VO>
-      <<<dim3(cdiv(g.ix, xtile<ORDER, VISC, VO>()), cdiv(g.iy, XROWS), g.iz), blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd, C.variant, cx);
+  k_sweep_x<R, S, ORDER, SCHEME, VISC, VO><<<dim3(cdiv(g.ix, xtile<ORDER, VISC, VO>()), cdiv(g.iy, XROWS), g.iz),
+                                             blk, 0, C.stream>>>(P, (S*)C.Racc, g, cr, C.tvd, C.variant, cx);
   prof_mark(C, "sweep_x");
   k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 1, EP_RHS>
-      <<<dim3(cdiv(g.ix, 64), cdiv(g.iz, 4), cdiv(g.iy, LSEG)), blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd,
-                                                                                    C.variant, cy, sa);
+      <<<dim3(cdiv(g.ix, 64), cdiv(g.iz, 4), cdiv(g.iy, LSEG)), blk, 0, C.stream>>>(P, (S*)C.Racc, g, cr, C.tvd,
+                                                                                    C.variant, cy, sa, Pn, c64);
   prof_mark(C, "sweep_y");
   const dim3 gz(cdiv(g.ix, 64), cdiv(g.iy, 4), cdiv(g.iz, LSEG));
 #define OX_Z(E)                                                                                                 \
-  k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 2, E><<<gz, blk, 0, C.stream>>>(P, C.Racc, g, cr, C.tvd, C.variant, \
-                                                                              cz, sa)
+  k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 2, E><<<gz, blk, 0, C.stream>>>(P, (S*)C.Racc, g, cr, C.tvd, C.variant, \
+                                                                              cz, sa, Pn, c64)
   switch (sa.ep) {
     case EP_RK3_S1: OX_Z(EP_RK3_S1); break;
     case EP_RK3_S23: OX_Z(EP_RK3_S23); break;
@@ -61,8 +62,9 @@ void run_stage(Core& C, const double* Qsrc, const StageArgs& sa, double cx, doub
 #undef OX_Z
   prof_mark(C, "sweep_z_rk");
   if (sa.ep != EP_RHS) {
-    launch_ghost_fill(C, sa.Qout);
-    prof_mark(C, "ghost_fill");
+    // the epilogue wrote the primitives of sa.Qout into the other set
+    C.cur = 1 - C.cur;
+    C.prim_for = sa.Qout;
   }
 }
 

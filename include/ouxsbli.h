@@ -177,6 +177,82 @@ typedef struct ox_kernel_time {
 int ox_set_profiling(ox_solver* s, int32_t enable);
 int ox_kernel_times(ox_solver* s, ox_kernel_time* out, int32_t cap, int32_t* n);
 
+/* Device bytes ox_create allocates for this configuration (state, stage
+ * buffers, primitives, scratch): the admission measure of the batch engine.
+ * No reference counterpart (the reference runs one case per executable). */
+int ox_footprint(const ox_config* cfg, uint64_t* bytes);
+
+/* Picks the y- and z-sweep segment lengths for this handle's grid by timing
+ * candidate values (16, 32, 64, 128 cells) on right-hand-side evaluations of
+ * the current state, keeps the fastest in the handle and returns them.  The
+ * state is not changed.  No reference counterpart (the reference's launch
+ * shapes are fixed at build time, calc_flux_base.f90.fypp:44-63). */
+int ox_autotune(ox_solver* s, int32_t* lseg_y, int32_t* lseg_z);
+
+/* ---------------------------------------------------------------------------
+ * Batch engine: many cases resident on one GPU, advanced together.  Cases
+ * that share a kernel specialisation and RK scheme form a group; one RK stage
+ * of a group is one persistent launch per sweep over the tiles of all its
+ * cases, and one step of the whole batch is replayed from a captured HIP
+ * graph while the set of cases does not change.  Cases join (ox_batch_admit)
+ * and leave (when their step count is reached) between steps.  The reference
+ * has no counterpart: it runs one case per `mpirun` (ouxsbli/case.py:180-201).
+ * Each case's result is bit-identical to advancing it alone with ox_step.
+ * A batch is driven from one host thread.
+ * ------------------------------------------------------------------------- */
+typedef struct ox_batch ox_batch;
+
+typedef struct ox_batch_config {
+  int32_t abi_version;   /* OX_ABI_VERSION                                    */
+  int32_t device;
+  int32_t max_cases;     /* resident cases at most (slots)                    */
+  int32_t use_graph;     /* replay each step from a captured HIP graph        */
+  int32_t autotune;      /* tune segment lengths per (specialisation, grid)   */
+  int32_t pad;
+  uint64_t hbm_budget;   /* device bytes the resident cases may hold          */
+} ox_batch_config;
+
+typedef struct ox_batch_stats {
+  int32_t active;        /* cases still stepping                               */
+  int32_t done;          /* finished cases not yet retired                     */
+  int32_t groups;        /* kernel-specialisation groups among active cases    */
+  int32_t graph_captures;
+  int64_t steps;         /* batch steps run                                    */
+  int64_t graph_replays;
+  uint64_t bytes_in_use; /* footprint of resident (active + done) cases        */
+  double cell_stages;    /* interior cell updates x RK stages, all cases       */
+  double device_ms;      /* device time of all batch steps (HIP events)        */
+} ox_batch_stats;
+
+int ox_batch_create(const ox_batch_config* cfg, ox_batch** out);
+void ox_batch_destroy(ox_batch* b);
+
+/* Admits one case: physical state Q(nx,5,ny,nz) (ghosts included), advanced
+ * by nsteps RK steps in later ox_batch_step calls.  Returns OX_EBUSY, with
+ * nothing allocated, when the case does not fit in the free slots or the
+ * free HBM budget now. */
+int ox_batch_admit(ox_batch* b, const ox_config* cfg, const double* Q, int32_t nsteps, int64_t* id);
+
+/* Fills the engine's segment-length cache for cases of this configuration
+ * (grid and specialisation) by tuning on the state Q, without admitting
+ * anything: lets a server tune ahead of a request stream. */
+int ox_batch_tune(ox_batch* b, const ox_config* cfg, const double* Q);
+
+/* One RK step of every active case (all groups), then blocks until it has
+ * finished.  Cases whose step count is reached move to the done list.
+ * *active receives the number of cases still stepping. */
+int ox_batch_step(ox_batch* b, int32_t* active);
+
+/* Ids of finished cases not yet retired (at most cap, oldest first). */
+int ox_batch_done(ox_batch* b, int64_t* ids, int32_t cap, int32_t* n);
+
+/* Copies a finished case's physical state Q(nx,5,ny,nz) out (ghost layers
+ * filled) and frees the case.  OX_ENOTFOUND for an unknown id, OX_ESTATE for
+ * a case still stepping. */
+int ox_batch_retire(ox_batch* b, int64_t id, double* Q);
+
+int ox_batch_query(ox_batch* b, ox_batch_stats* out);
+
 /* Thread-local text of the last error on this thread ("" if none). */
 const char* ox_last_error(void);
 

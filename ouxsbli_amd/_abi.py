@@ -82,6 +82,39 @@ class OxKernelTime(ctypes.Structure):
     ]
 
 
+class OxBatchConfig(ctypes.Structure):
+    """``ox_batch_config``."""
+
+    _fields_ = [
+        ("abi_version", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("max_cases", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+        ("autotune", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("hbm_budget", ctypes.c_uint64),
+    ]
+
+
+class OxBatchStats(ctypes.Structure):
+    """``ox_batch_stats``."""
+
+    _fields_ = [
+        ("active", ctypes.c_int32),
+        ("done", ctypes.c_int32),
+        ("groups", ctypes.c_int32),
+        ("graph_captures", ctypes.c_int32),
+        ("steps", ctypes.c_int64),
+        ("graph_replays", ctypes.c_int64),
+        ("bytes_in_use", ctypes.c_uint64),
+        ("cell_stages", ctypes.c_double),
+        ("device_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 # Every symbol include/ouxsbli.h declares (checked by tests/test_abi.py).
 EXPORTS = (
     "ox_create",
@@ -98,6 +131,16 @@ EXPORTS = (
     "ox_last_step_ms",
     "ox_set_profiling",
     "ox_kernel_times",
+    "ox_footprint",
+    "ox_autotune",
+    "ox_batch_create",
+    "ox_batch_destroy",
+    "ox_batch_admit",
+    "ox_batch_tune",
+    "ox_batch_step",
+    "ox_batch_done",
+    "ox_batch_retire",
+    "ox_batch_query",
     "ox_last_error",
     "ox_version",
 )

@@ -61,6 +61,26 @@ def load() -> ctypes.CDLL:
     L.ox_kernel_times.argtypes = [P, ctypes.POINTER(_abi.OxKernelTime), ctypes.c_int32,
                                   ctypes.POINTER(ctypes.c_int32)]
     L.ox_kernel_times.restype = I
+    L.ox_footprint.argtypes = [ctypes.POINTER(_abi.OxConfig), ctypes.POINTER(ctypes.c_uint64)]
+    L.ox_footprint.restype = I
+    L.ox_autotune.argtypes = [P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+    L.ox_autotune.restype = I
+    L.ox_batch_create.argtypes = [ctypes.POINTER(_abi.OxBatchConfig), ctypes.POINTER(P)]
+    L.ox_batch_create.restype = I
+    L.ox_batch_destroy.argtypes = [P]
+    L.ox_batch_destroy.restype = None
+    L.ox_batch_admit.argtypes = [P, ctypes.POINTER(_abi.OxConfig), D, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]
+    L.ox_batch_admit.restype = I
+    L.ox_batch_tune.argtypes = [P, ctypes.POINTER(_abi.OxConfig), D]
+    L.ox_batch_tune.restype = I
+    L.ox_batch_step.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
+    L.ox_batch_step.restype = I
+    L.ox_batch_done.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+    L.ox_batch_done.restype = I
+    L.ox_batch_retire.argtypes = [P, ctypes.c_int64, D]
+    L.ox_batch_retire.restype = I
+    L.ox_batch_query.argtypes = [P, ctypes.POINTER(_abi.OxBatchStats)]
+    L.ox_batch_query.restype = I
     L.ox_last_error.argtypes = []
     L.ox_last_error.restype = ctypes.c_char_p
     L.ox_version.argtypes = []

@@ -1,0 +1,114 @@
+"""Python handle over the batch engine of include/ouxsbli.h (ox_batch_*).
+
+Many cases are resident on the GPU at once; every ``step()`` advances all of
+them by one RK step, with one persistent launch per sweep and RK stage for
+each group of cases that share a kernel specialisation, replayed from a
+captured HIP graph while the set of cases is unchanged.  Cases join with
+``admit`` (refused with ``Busy`` while they do not fit in the free slots or
+HBM budget) and leave when their step count is reached (``done`` /
+``retire``).  The reference has no counterpart: it runs one case per
+``mpirun`` (ouxsbli/case.py:180-201).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _abi, _native
+from .cases import CaseSpec
+
+
+class Busy(Exception):
+    """Admission refused for now (no free slot or HBM budget)."""
+
+
+def footprint(spec: CaseSpec) -> int:
+    """Device bytes a case of this specification holds while resident."""
+    b = ctypes.c_uint64()
+    cfg = spec.to_ox()
+    _native.check(_native.load().ox_footprint(ctypes.byref(cfg), ctypes.byref(b)))
+    return int(b.value)
+
+
+class Batch:
+    def __init__(self, hbm_budget: int, max_cases: int = 64, use_graph: bool = True, autotune: bool = True,
+                 device: int = 0):
+        self._lib = _native.load()
+        c = _abi.OxBatchConfig()
+        c.abi_version = _abi.OX_ABI_VERSION
+        c.device = device
+        c.max_cases = max_cases
+        c.use_graph = int(use_graph)
+        c.autotune = int(autotune)
+        c.hbm_budget = int(hbm_budget)
+        h = ctypes.c_void_p()
+        _native.check(self._lib.ox_batch_create(ctypes.byref(c), ctypes.byref(h)))
+        self._h = h
+        self._shapes = {}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.ox_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, spec: CaseSpec, Q: np.ndarray, nsteps: int) -> int:
+        """Admits a case (physical state Q, numpy shape (nz, ny, 5, nx)) for
+        nsteps RK steps; returns its id.  Raises Busy if it does not fit now."""
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        if Q.shape != spec.state_shape():
+            raise ValueError(f"state shape {Q.shape} != {spec.state_shape()}")
+        cfg = spec.to_ox()
+        cid = ctypes.c_int64()
+        rc = self._lib.ox_batch_admit(self._h, ctypes.byref(cfg), Q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      int(nsteps), ctypes.byref(cid))
+        if rc == _abi.OX_EBUSY:
+            raise Busy(self._lib.ox_last_error().decode())
+        _native.check(rc)
+        self._shapes[cid.value] = spec.state_shape()
+        return cid.value
+
+    def tune(self, spec: CaseSpec, Q: np.ndarray) -> None:
+        """Tunes (and caches) the segment lengths of cases like ``spec``
+        without admitting anything."""
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        cfg = spec.to_ox()
+        _native.check(self._lib.ox_batch_tune(self._h, ctypes.byref(cfg),
+                                              Q.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def step(self) -> int:
+        """One RK step of every active case; returns how many are still stepping."""
+        n = ctypes.c_int32()
+        _native.check(self._lib.ox_batch_step(self._h, ctypes.byref(n)))
+        return n.value
+
+    def done(self) -> list:
+        cap = 1024
+        ids = (ctypes.c_int64 * cap)()
+        n = ctypes.c_int32()
+        _native.check(self._lib.ox_batch_done(self._h, ids, cap, ctypes.byref(n)))
+        return [ids[k] for k in range(n.value)]
+
+    def retire(self, cid: int) -> np.ndarray:
+        """Final physical state of a finished case (ghost layers filled); frees it."""
+        Q = np.empty(self._shapes[cid], dtype=np.float64)
+        _native.check(self._lib.ox_batch_retire(self._h, int(cid), Q.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        del self._shapes[cid]
+        return Q
+
+    def stats(self) -> dict:
+        st = _abi.OxBatchStats()
+        _native.check(self._lib.ox_batch_query(self._h, ctypes.byref(st)))
+        return st.as_dict()

@@ -4,7 +4,9 @@ Each translation unit is compiled with hipcc for gfx950 into an object, then
 linked into one shared library that exports the C ABI of include/ouxsbli.h.
 The FP64 units are compiled with -ffp-contract=off (every multiply and add
 rounded separately, in the evaluation order of the reference expressions);
-the FP32-bulk unit allows contraction.  Packed FP32 instructions are disabled
+the FP32-bulk units contract a*b+c within one source expression
+(-ffp-contract=on), a rule that does not depend on the surrounding code, so
+the single-case and the batched launches of a case round identically.  Packed FP32 instructions are disabled
 for all device code.
 """
 from __future__ import annotations
@@ -31,8 +33,11 @@ COMMON = [
 ]
 UNITS = {
     "ox_capi.hip": ["-ffp-contract=off"],
+    "ox_batch.hip": ["-ffp-contract=off"],
     "ox_fp64.hip": ["-ffp-contract=off"],
-    "ox_fp32.hip": ["-ffp-contract=fast"],
+    "ox_fp64_batch.hip": ["-ffp-contract=off"],
+    "ox_fp32.hip": ["-ffp-contract=on"],
+    "ox_fp32_batch.hip": ["-ffp-contract=on"],
 }
 
 

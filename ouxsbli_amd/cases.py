@@ -192,6 +192,8 @@ def tgv_state(spec: CaseSpec) -> np.ndarray:
 def initial_state(spec: CaseSpec) -> np.ndarray:
     if spec.ic == "tgv":
         return tgv_state(spec)
+    if spec.ic == "khi_tanh":
+        return khi_state(spec)
     raise ValueError(f"unknown initial condition {spec.ic!r}")
 
 
@@ -206,3 +208,49 @@ def config2_spec(n: int = 256, precision: str = "fp64") -> CaseSpec:
     central-6 viscous, RK3-TVD, CFL 0.03 (the NSTGV case file)."""
     return tgv_spec(n, 6, Ma=1.25, cfl=0.03, scheme="SLAU", visc="NS", rk=3, tvd="hybrid",
                     slau_variant="HRSLAU2", threshold=0.4, precision=precision)
+
+
+# ---------------------------------------------------------------------------
+# Serving workload: 3-D Kelvin-Helmholtz solves (the KHI case family of
+# 3D_solver/KHI: Euler, RK3, no TVD, HR-SLAU2 for SLAU), here with a periodic
+# double tanh shear layer of thickness L/28 at Mach 0.5 and seeded
+# velocity noise of 1e-3 U, on [0, 1)^3 in non-dimensional units
+# (rho0 = 1, p0 = 1/gamma, so c0 = 1).
+# ---------------------------------------------------------------------------
+def khi_spec(n: int, order: int, scheme: str, precision: str = "fp64", seed: int = 0, Ma: float = 0.5,
+             cfl: float = 0.3) -> CaseSpec:
+    N = n + order
+    L = 1.0
+    c0 = 1.0
+    U = Ma * c0
+    dt = cfl * (L / n) / (U + c0)
+    spec = CaseSpec(nx=N, ny=N, nz=N, order=order, scheme=scheme, visc="Euler", visc_order=0, tvd="none",
+                    slau_variant="HRSLAU2", rk=3, precision=precision, dt=dt, Lx=L, Ly=L, Lz=L, ic="khi_tanh",
+                    V0=U, RHO0=1.0, p0=1.0 / GAMMA, seed=seed)
+    return spec
+
+
+def khi_state(spec: CaseSpec) -> np.ndarray:
+    """Double tanh shear layer u(y) = U/2 * (tanh((y-1/4)/d) - tanh((y-3/4)/d) - 1),
+    d = L/28, plus seeded noise of amplitude 1e-3 U in u, v, w (uniform in
+    [-1, 1]); uniform density and pressure."""
+    x = cell_centres(spec.nx, spec.Lx, spec.order)
+    y = cell_centres(spec.ny, spec.Ly, spec.order)
+    z = cell_centres(spec.nz, spec.Lz, spec.order)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    U, d = spec.V0, spec.Ly / 28.0
+    rng = np.random.default_rng(spec.seed)
+    shape = X.shape
+    u = 0.5 * U * (np.tanh((Y - 0.25 * spec.Ly) / d) - np.tanh((Y - 0.75 * spec.Ly) / d) - 1.0)
+    u = u + 1e-3 * U * rng.uniform(-1.0, 1.0, shape)
+    v = 1e-3 * U * rng.uniform(-1.0, 1.0, shape)
+    w = 1e-3 * U * rng.uniform(-1.0, 1.0, shape)
+    rho = np.full(shape, spec.RHO0)
+    p = np.full(shape, spec.p0)
+    Q = np.zeros(spec.state_shape(), dtype=np.float64)
+    Q[:, :, 0, :] = rho
+    Q[:, :, 1, :] = rho * u
+    Q[:, :, 2, :] = rho * v
+    Q[:, :, 3, :] = rho * w
+    Q[:, :, 4, :] = p / (spec.gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+    return periodic_fill(Q, spec.order)

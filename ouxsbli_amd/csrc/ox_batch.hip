@@ -1,0 +1,545 @@
+// ox_batch.hip -- the batch engine of include/ouxsbli.h (ox_batch_*) and the
+// segment-length autotuner (ox_autotune).
+//
+// Every admitted case is an ox_solver whose kernels run on the engine's
+// stream.  Cases with the same kernel specialisation, precision and RK scheme
+// form a group; one RK stage of a group is three persistent launches (x, y, z
+// sweeps) over the concatenated tiles of its cases, each case's tiles reading
+// that case's descriptor (SweepArgs).  The descriptors of a whole step are
+// built on the host (each case's primitive-set parity advances stage by
+// stage), copied to the device in one transfer, and the launches of the step
+// are replayed from a HIP graph captured for the current set of cases.
+// A case's arithmetic is the same code on the same data as its single-case
+// launches, so its result does not depend on what else is in the batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "ox_core.hpp"
+#include "ox_kernels.hpp"
+#include "ox_solver.hpp"
+
+extern "C" {
+int ox_create(const ox_config* cfg, ox_solver** out);
+void ox_destroy(ox_solver* s);
+int ox_set_state(ox_solver* s, const double* Q);
+int ox_get_state(ox_solver* s, double* Q);
+int ox_autotune(ox_solver* s, int32_t* lseg_y, int32_t* lseg_z);
+}
+
+namespace {
+
+constexpr int kMaxGrid = 1024;  // persistent blocks per launch (4 per CU on 256 CUs)
+constexpr size_t kMaxArgs = 2048;
+static_assert(sizeof(ox::SweepArgs<double, double>) <= kMaxArgs, "sweep descriptor size");
+static_assert(sizeof(ox::SweepArgs<float, float>) <= kMaxArgs, "sweep descriptor size");
+
+struct BCase {
+  int64_t id;
+  ox_solver* s;
+  int remaining;
+  size_t bytes;
+  int group;  // index into ox_batch::groups while active
+};
+
+struct BGroup {
+  int prec, key, rk;
+  ox::BatchOps ops;
+  std::vector<BCase*> members;
+};
+
+unsigned cdivu(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace
+
+struct ox_batch {
+  ox_batch_config cfg{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<std::unique_ptr<BCase>> active;  // admission order
+  std::deque<std::unique_ptr<BCase>> done;
+  std::vector<BGroup> groups;
+  int64_t next_id = 1;
+  uint64_t bytes_in_use = 0;
+  // per-step launch plan (rebuilt when the set of active cases changes)
+  bool plan_valid = false;
+  std::vector<ox::BatchGroupLaunch> launches;  // [stage-major: for each stage, for each group]
+  std::vector<int> launch_stage;
+  std::vector<int> launch_grp;
+  std::vector<size_t> args_off;                // per launch: byte offset of its x,y,z descriptor arrays
+  size_t args_bytes = 0;
+  char* d_args = nullptr;
+  size_t d_args_cap = 0;
+  char* h_args = nullptr;  // pinned staging of the descriptors
+  size_t h_args_cap = 0;
+  ox::TileMap* d_map = nullptr;
+  size_t d_map_cap = 0;
+  // captured launches of one step for the current set of cases (the
+  // descriptors they read are refreshed before every replay)
+  hipGraphExec_t graph = nullptr;
+  // autotune cache: (precision, stage key, nx, ny, nz) -> (lseg_y, lseg_z)
+  std::map<std::tuple<int, int, int, int, int>, std::pair<int, int>> tuned;
+  // statistics
+  int64_t steps = 0, graph_replays = 0;
+  int32_t graph_captures = 0;
+  double cell_stages = 0.0, device_ms = 0.0;
+};
+
+namespace {
+
+void drop_graphs(ox_batch* b) {
+  if (b->graph) hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+}
+
+void invalidate(ox_batch* b) {
+  b->plan_valid = false;
+  drop_graphs(b);
+}
+
+int ensure_device(char** p, size_t* cap, size_t need) {
+  if (need <= *cap) return OX_OK;
+  if (*p) hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (hipMalloc(p, need) != hipSuccess) return ox_fail(OX_ENOMEM, "batch descriptor allocation failed");
+  *cap = need;
+  return OX_OK;
+}
+
+// Groups and the per-step launch plan for the current active cases: tile
+// maps (uploaded here), descriptor offsets, persistent grid sizes.
+int build_plan(ox_batch* b) {
+  b->groups.clear();
+  for (auto& up : b->active) {
+    BCase* c = up.get();
+    const ox::Core& C = c->s->C;
+    const int prec = C.cfg.precision, key = ox::stage_key(C.cfg.order, C.cfg.scheme, C.cfg.visc, C.vorder);
+    int gi = -1;
+    for (size_t g = 0; g < b->groups.size(); ++g)
+      if (b->groups[g].prec == prec && b->groups[g].key == key && b->groups[g].rk == C.cfg.rk) gi = (int)g;
+    if (gi < 0) {
+      BGroup G;
+      G.prec = prec;
+      G.key = key;
+      G.rk = C.cfg.rk;
+      const bool ok = prec == OX_PREC_FP64 ? ox::find_batch_fp64(C.cfg.order, C.cfg.scheme, C.cfg.visc, C.vorder, &G.ops)
+                                           : ox::find_batch_fp32(C.cfg.order, C.cfg.scheme, C.cfg.visc, C.vorder, &G.ops);
+      if (!ok) return ox_fail(OX_EUNSUPPORTED, "no batched specialisation for a case");
+      b->groups.push_back(G);
+      gi = (int)b->groups.size() - 1;
+    }
+    c->group = gi;
+    b->groups[gi].members.push_back(c);
+  }
+  // tile maps: per group, per direction, one TileMap per member
+  std::vector<ox::TileMap> maps;
+  std::vector<size_t> map_off;  // per group: index of its x map (y at +n, z at +2n)
+  std::vector<std::array<int, 3>> ntiles;
+  for (BGroup& G : b->groups) {
+    map_off.push_back(maps.size());
+    std::array<int, 3> nt{0, 0, 0};
+    for (int d = 0; d < 3; ++d) {
+      for (BCase* c : G.members) {
+        const ox::Geom& g = c->s->C.g;
+        const int* ls = c->s->C.lseg;
+        ox::TileMap m{};
+        unsigned nbx, nby, nbz;
+        if (d == 0) {
+          nbx = cdivu(g.ix, G.ops.xt); nby = cdivu(g.iy, ox::XROWS); nbz = g.iz;
+        } else if (d == 1) {
+          nbx = cdivu(g.ix, 64); nby = cdivu(g.iz, 4); nbz = cdivu(g.iy, ls[0]);
+        } else {
+          nbx = cdivu(g.ix, 64); nby = cdivu(g.iy, 4); nbz = cdivu(g.iz, ls[1]);
+        }
+        m.start = nt[d];
+        m.nbx = (int)nbx;
+        m.nby = (int)nby;
+        nt[d] += (int)(nbx * nby * nbz);
+        maps.push_back(m);
+      }
+    }
+    ntiles.push_back(nt);
+  }
+  if (!maps.empty()) {
+    int rc = ensure_device((char**)&b->d_map, &b->d_map_cap, maps.size() * sizeof(ox::TileMap));
+    if (rc != OX_OK) return rc;
+    OX_HIP(hipMemcpyAsync(b->d_map, maps.data(), maps.size() * sizeof(ox::TileMap), hipMemcpyHostToDevice, b->stream));
+    OX_HIP(hipStreamSynchronize(b->stream));
+  }
+  // launches: for each stage (up to 4), for each group with that many stages
+  b->launches.clear();
+  b->launch_stage.clear();
+  b->launch_grp.clear();
+  b->args_off.clear();
+  size_t off = 0;
+  for (int st = 0; st < 4; ++st) {
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+      const BGroup& G = b->groups[gi];
+      if (st >= G.rk) continue;
+      const int n = (int)G.members.size();
+      ox::BatchGroupLaunch L{};
+      L.stream = b->stream;
+      L.ncase = n;
+      for (int d = 0; d < 3; ++d) {
+        L.map[d] = b->d_map + map_off[gi] + (size_t)d * n;
+        L.ntiles[d] = ntiles[gi][d];
+        L.grid[d] = std::max(1, std::min(ntiles[gi][d], kMaxGrid));
+      }
+      b->launches.push_back(L);
+      b->launch_stage.push_back(st);
+      b->launch_grp.push_back((int)gi);
+      b->args_off.push_back(off);
+      off += 3 * (size_t)n * G.ops.args_bytes;
+      off = (off + 255) & ~(size_t)255;
+    }
+  }
+  b->args_bytes = off;
+  int rc = ensure_device(&b->d_args, &b->d_args_cap, std::max<size_t>(off, 256));
+  if (rc != OX_OK) return rc;
+  if (off > b->h_args_cap) {
+    if (b->h_args) hipHostFree(b->h_args);
+    b->h_args = nullptr;
+    b->h_args_cap = 0;
+    if (hipHostMalloc((void**)&b->h_args, off, hipHostMallocDefault) != hipSuccess)
+      return ox_fail(OX_ENOMEM, "pinned staging allocation failed");
+    b->h_args_cap = off;
+  }
+  for (size_t li = 0; li < b->launches.size(); ++li) {
+    const BGroup& G = b->groups[b->launch_grp[li]];
+    const size_t n = G.members.size();
+    for (int d = 0; d < 3; ++d)
+      b->launches[li].args[d] = b->d_args + b->args_off[li] + (size_t)d * n * G.ops.args_bytes;
+    b->launches[li].ep = 0;  // set per step
+  }
+  b->plan_valid = true;
+  return OX_OK;
+}
+
+// Segment lengths of a case from the engine's cache, tuning on a miss.
+int tune_case(ox_batch* b, ox_solver* s) {
+  const ox_config& cfg = s->C.cfg;
+  const auto key = std::make_tuple((int)cfg.precision, ox::stage_key(cfg.order, cfg.scheme, cfg.visc, s->C.vorder),
+                                   (int)cfg.nx, (int)cfg.ny, (int)cfg.nz);
+  auto it = b->tuned.find(key);
+  if (it != b->tuned.end()) {
+    s->C.lseg[0] = it->second.first;
+    s->C.lseg[1] = it->second.second;
+    return OX_OK;
+  }
+  int32_t ly, lz;
+  const int rc = ox_autotune(s, &ly, &lz);  // leaves the state unchanged
+  if (rc == OX_OK) b->tuned[key] = {ly, lz};
+  return rc;
+}
+
+void enqueue_step(ox_batch* b) {
+  for (size_t li = 0; li < b->launches.size(); ++li) b->groups[b->launch_grp[li]].ops.launch(b->launches[li]);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ox_footprint(const ox_config* cfg, uint64_t* bytes) {
+  if (!cfg || !bytes) return ox_fail(OX_EINVAL, "null argument");
+  *bytes = solver_footprint(*cfg);
+  return OX_OK;
+}
+
+int ox_autotune(ox_solver* s, int32_t* lseg_y, int32_t* lseg_z) {
+  if (!s) return ox_fail(OX_EINVAL, "null handle");
+  OX_HIP(hipSetDevice(s->C.cfg.device));
+  ox::Core& C = s->C;
+  const ox::Geom& g = C.g;
+  const bool prof = C.prof;
+  C.prof = false;
+  ox::StageArgs sa{s->QJ, s->QJ2, s->Rs, 0, 0, 0, ox::EP_RHS};
+  hipEvent_t e0, e1;
+  OX_HIP(hipEventCreate(&e0));
+  OX_HIP(hipEventCreate(&e1));
+  auto time_rhs = [&](int ly, int lz) -> float {
+    C.lseg[0] = ly;
+    C.lseg[1] = lz;
+    s->stage(C, s->QJ, sa, g.a_yz, g.a_zx, g.a_xy);  // warm-up
+    hipEventRecord(e0, C.stream);
+    for (int r = 0; r < 3; ++r) s->stage(C, s->QJ, sa, g.a_yz, g.a_zx, g.a_xy);
+    hipEventRecord(e1, C.stream);
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    return ms;
+  };
+  static const int cand[] = {16, 32, 64, 128};
+  int by = ox::LSEG_DEFAULT, bz = ox::LSEG_DEFAULT;
+  float best = 1e30f;
+  for (int ly : cand) {
+    if (ly > 2 * g.iy && ly != cand[0]) continue;
+    const float t = time_rhs(ly, bz);
+    if (t < best) best = t, by = ly;
+  }
+  best = 1e30f;
+  for (int lz : cand) {
+    if (lz > 2 * g.iz && lz != cand[0]) continue;
+    const float t = time_rhs(by, lz);
+    if (t < best) best = t, bz = lz;
+  }
+  C.lseg[0] = by;
+  C.lseg[1] = bz;
+  C.prof = prof;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  OX_HIP(hipGetLastError());
+  if (lseg_y) *lseg_y = by;
+  if (lseg_z) *lseg_z = bz;
+  return OX_OK;
+}
+
+int ox_batch_create(const ox_batch_config* cfg, ox_batch** out) {
+  if (!cfg || !out) return ox_fail(OX_EINVAL, "null argument");
+  *out = nullptr;
+  if (cfg->abi_version != OX_ABI_VERSION) return ox_fail(OX_EINVAL, "abi_version mismatch");
+  if (cfg->max_cases < 1) return ox_fail(OX_EINVAL, "max_cases must be >= 1");
+  ox_batch* b = new (std::nothrow) ox_batch();
+  if (!b) return ox_fail(OX_ENOMEM, "host allocation failed");
+  b->cfg = *cfg;
+  if (hipSetDevice(cfg->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+    ox_batch_destroy(b);
+    return ox_fail(OX_EDEVICE, "stream/event creation failed");
+  }
+  *out = b;
+  return OX_OK;
+}
+
+void ox_batch_destroy(ox_batch* b) {
+  if (!b) return;
+  hipSetDevice(b->cfg.device);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  drop_graphs(b);
+  auto free_case = [](BCase* c) {
+    c->s->C.stream = c->s->own_stream;
+    ox_destroy(c->s);
+  };
+  for (auto& c : b->active) free_case(c.get());
+  for (auto& c : b->done) free_case(c.get());
+  if (b->d_args) hipFree(b->d_args);
+  if (b->d_map) hipFree(b->d_map);
+  if (b->h_args) hipHostFree(b->h_args);
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int ox_batch_admit(ox_batch* b, const ox_config* cfg, const double* Q, int32_t nsteps, int64_t* id) {
+  if (!b || !cfg || !Q || !id) return ox_fail(OX_EINVAL, "null argument");
+  if (nsteps < 1) return ox_fail(OX_EINVAL, "nsteps must be >= 1");
+  if (cfg->device != b->cfg.device) return ox_fail(OX_EINVAL, "case device differs from the batch device");
+  if (cfg->visc == OX_VISC_LES) return ox_fail(OX_EUNSUPPORTED, "LES cases are not batched (use ox_step)");
+  const size_t need = solver_footprint(*cfg);
+  if ((int)(b->active.size() + b->done.size()) >= b->cfg.max_cases) return ox_fail(OX_EBUSY, "no free slot");
+  if (b->bytes_in_use + need > b->cfg.hbm_budget) return ox_fail(OX_EBUSY, "footprint exceeds the free HBM budget");
+  ox_solver* s = nullptr;
+  int rc = ox_create(cfg, &s);
+  if (rc != OX_OK) return rc;
+  rc = ox_set_state(s, Q);
+  if (rc == OX_OK && b->cfg.autotune) rc = tune_case(b, s);
+  if (rc != OX_OK) {
+    ox_destroy(s);
+    return rc;
+  }
+  // primitives of the initial state on the handle's own stream, then the
+  // case joins the engine's stream
+  {
+    ox::BatchOps ops;
+    const bool ok = cfg->precision == OX_PREC_FP64
+                        ? ox::find_batch_fp64(cfg->order, cfg->scheme, cfg->visc, s->C.vorder, &ops)
+                        : ox::find_batch_fp32(cfg->order, cfg->scheme, cfg->visc, s->C.vorder, &ops);
+    if (!ok) {
+      ox_destroy(s);
+      return ox_fail(OX_EUNSUPPORTED, "no batched specialisation for this case");
+    }
+    if (s->C.prim_for != s->QJ) ops.prepare(s->C, s->QJ);  // k_prim
+    if (hipStreamSynchronize(s->C.stream) != hipSuccess) {
+      ox_destroy(s);
+      return ox_fail(OX_EDEVICE, "case initialisation failed");
+    }
+  }
+  s->C.stream = b->stream;
+  s->C.prof = false;
+  auto c = std::make_unique<BCase>();
+  c->id = b->next_id++;
+  c->s = s;
+  c->remaining = nsteps;
+  c->bytes = need;
+  c->group = -1;
+  b->bytes_in_use += need;
+  *id = c->id;
+  b->active.push_back(std::move(c));
+  invalidate(b);
+  return OX_OK;
+}
+
+int ox_batch_tune(ox_batch* b, const ox_config* cfg, const double* Q) {
+  if (!b || !cfg || !Q) return ox_fail(OX_EINVAL, "null argument");
+  ox_solver* s = nullptr;
+  int rc = ox_create(cfg, &s);
+  if (rc != OX_OK) return rc;
+  rc = ox_set_state(s, Q);
+  if (rc == OX_OK) rc = tune_case(b, s);
+  ox_destroy(s);
+  return rc;
+}
+
+int ox_batch_step(ox_batch* b, int32_t* active) {
+  if (!b) return ox_fail(OX_EINVAL, "null handle");
+  OX_HIP(hipSetDevice(b->cfg.device));
+  if (b->active.empty()) {
+    if (active) *active = 0;
+    return OX_OK;
+  }
+  if (!b->plan_valid) {
+    int rc = build_plan(b);
+    if (rc != OX_OK) return rc;
+  }
+  // descriptors of every stage of the step; each case's primitive-set parity
+  // advances stage by stage exactly as in ox_step
+  for (size_t li = 0; li < b->launches.size(); ++li) {
+    BGroup& G = b->groups[b->launch_grp[li]];
+    const int st = b->launch_stage[li];
+    const size_t n = G.members.size(), ab = G.ops.args_bytes;
+    char* base = b->h_args + b->args_off[li];
+    for (size_t m = 0; m < n; ++m) {
+      ox_solver* s = G.members[m]->s;
+      const ox::Geom& g = s->C.g;
+      const double* Qsrc;
+      ox::StageArgs sa;
+      double coef;
+      rk_stage(s, st, &Qsrc, &sa, &coef);
+      if (s->C.prim_for != Qsrc) return ox_fail(OX_ESTATE, "batched case without current primitives");
+      alignas(16) char tmp[3 * kMaxArgs];
+      G.ops.make_args(s->C, sa, coef * g.a_yz, coef * g.a_zx, coef * g.a_xy, tmp);
+      for (int d = 0; d < 3; ++d) std::memcpy(base + ((size_t)d * n + m) * ab, tmp + (size_t)d * ab, ab);
+      b->launches[li].ep = sa.ep;
+      ox::finish_stage(s->C, sa);
+    }
+  }
+  OX_HIP(hipMemcpyAsync(b->d_args, b->h_args, b->args_bytes, hipMemcpyHostToDevice, b->stream));
+  OX_HIP(hipEventRecord(b->ev0, b->stream));
+  bool replayed = false;
+  if (b->cfg.use_graph) {
+    if (!b->graph) {
+      // capture the step's launches once for this set of cases
+      hipGraph_t graph = nullptr;
+      OX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_step(b);
+      OX_HIP(hipStreamEndCapture(b->stream, &graph));
+      const hipError_t e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        b->graph = nullptr;
+        return ox_fail(OX_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+      ++b->graph_captures;
+    }
+    OX_HIP(hipGraphLaunch(b->graph, b->stream));
+    ++b->graph_replays;
+    replayed = true;
+  }
+  if (!replayed) enqueue_step(b);
+  OX_HIP(hipGetLastError());
+  OX_HIP(hipEventRecord(b->ev1, b->stream));
+  OX_HIP(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  OX_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->device_ms += ms;
+  ++b->steps;
+  // bookkeeping: stage counts, finished cases
+  bool changed = false;
+  for (size_t a = 0; a < b->active.size();) {
+    BCase* c = b->active[a].get();
+    const ox::Geom& g = c->s->C.g;
+    b->cell_stages += (double)g.ix * g.iy * g.iz * c->s->C.cfg.rk;
+    if (--c->remaining == 0) {
+      ox::launch_ghost_fill(c->s->C, c->s->QJ);  // the result is read with its ghost layers
+      b->done.push_back(std::move(b->active[a]));
+      b->active.erase(b->active.begin() + a);
+      changed = true;
+    } else {
+      ++a;
+    }
+  }
+  if (changed) {
+    OX_HIP(hipStreamSynchronize(b->stream));
+    invalidate(b);
+  }
+  if (active) *active = (int32_t)b->active.size();
+  return OX_OK;
+}
+
+int ox_batch_done(ox_batch* b, int64_t* ids, int32_t cap, int32_t* n) {
+  if (!b || !n) return ox_fail(OX_EINVAL, "null argument");
+  int32_t k = 0;
+  for (auto& c : b->done) {
+    if (k < cap && ids) ids[k] = c->id;
+    ++k;
+  }
+  *n = std::min(k, cap);
+  return OX_OK;
+}
+
+int ox_batch_retire(ox_batch* b, int64_t id, double* Q) {
+  if (!b) return ox_fail(OX_EINVAL, "null handle");
+  for (auto& c : b->active)
+    if (c->id == id) return ox_fail(OX_ESTATE, "case is still stepping");
+  for (auto it = b->done.begin(); it != b->done.end(); ++it) {
+    if ((*it)->id != id) continue;
+    BCase* c = it->get();
+    int rc = OX_OK;
+    if (Q) rc = ox_get_state(c->s, Q);
+    c->s->C.stream = c->s->own_stream;
+    ox_destroy(c->s);
+    b->bytes_in_use -= c->bytes;
+    b->done.erase(it);
+    return rc;
+  }
+  return ox_fail(OX_ENOTFOUND, "unknown case id");
+}
+
+int ox_batch_query(ox_batch* b, ox_batch_stats* out) {
+  if (!b || !out) return ox_fail(OX_EINVAL, "null argument");
+  std::memset(out, 0, sizeof(*out));
+  out->active = (int32_t)b->active.size();
+  out->done = (int32_t)b->done.size();
+  int ng = 0;
+  {
+    std::vector<std::tuple<int, int, int>> seen;
+    for (auto& c : b->active) {
+      const ox::Core& C = c->s->C;
+      auto t = std::make_tuple((int)C.cfg.precision, ox::stage_key(C.cfg.order, C.cfg.scheme, C.cfg.visc, C.vorder),
+                               (int)C.cfg.rk);
+      if (std::find(seen.begin(), seen.end(), t) == seen.end()) seen.push_back(t);
+    }
+    ng = (int)seen.size();
+  }
+  out->groups = ng;
+  out->graph_captures = b->graph_captures;
+  out->steps = b->steps;
+  out->graph_replays = b->graph_replays;
+  out->bytes_in_use = b->bytes_in_use;
+  out->cell_stages = b->cell_stages;
+  out->device_ms = b->device_ms;
+  return OX_OK;
+}
+
+}  // extern "C"

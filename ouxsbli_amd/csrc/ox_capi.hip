@@ -12,6 +12,7 @@
 #include "ox_core.hpp"
 #include "ox_diag.hpp"
 #include "ox_kernels.hpp"
+#include "ox_solver.hpp"
 
 namespace ox {
 
@@ -71,17 +72,16 @@ __global__ __launch_bounds__(256) void k_scale(double* __restrict__ Q, long long
 
 }  // namespace ox
 
-// Per-kernel event profiling (ox_set_profiling / ox_kernel_times).
-struct ProfState {
-  std::vector<hipEvent_t> pool;       // events recorded this ox_step call
-  std::vector<const char*> names;     // name of the kernel that ends at pool[m] (m >= 1)
-  size_t used = 0;
-  std::vector<std::string> acc_names;
-  std::vector<double> acc_ms;
-  std::vector<long long> acc_n;
-};
+namespace {
+thread_local std::string g_err;
+}  // namespace
 
-static void prof_record(ox::Core* C, const char* name) {
+int ox_fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+void prof_record(ox::Core* C, const char* name) {
   ProfState* P = static_cast<ProfState*>(C->prof_state);
   if (P->used == P->pool.size()) {
     hipEvent_t e;
@@ -94,55 +94,63 @@ static void prof_record(ox::Core* C, const char* name) {
   ++P->used;
 }
 
-struct ox_solver {
-  ox::Core C;
-  ProfState prof;
-  double* QJ = nullptr;   // state Q^n
-  double* QJ2 = nullptr;  // QJ2 (RK3) / QJs (RK4)
-  double* Rs = nullptr;   // RK4 accumulator
-  double* scratch = nullptr;
-  size_t scratch_bytes = 0;
-  ox::StageFn stage = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double last_ms = 0.0;
-};
+// Stage `st` (0-based) of one RK step: its source state, epilogue and the
+// factor on dt * face area (calc_time_dev.f90.fypp:86-100, :268-285).
+void rk_stage(ox_solver* s, int st, const double** Qsrc, ox::StageArgs* sa, double* coef) {
+  const ox::Core& C = s->C;
+  if (C.cfg.rk == 3) {
+    if (st == 0) {
+      *sa = {s->QJ, s->QJ2, s->Rs, 0, 0, 0, ox::EP_RK3_S1};
+      *Qsrc = s->QJ;
+      *coef = 1.0;
+    } else if (st == 1) {
+      *sa = {s->QJ, s->QJ2, s->Rs, 0.75, 0.25, 1.0, ox::EP_RK3_S23};
+      *Qsrc = s->QJ2;
+      *coef = 0.25;
+    } else {
+      *sa = {s->QJ2, s->QJ, s->Rs, 2.0, 1.0, C.hc.one_third, ox::EP_RK3_S23};
+      *Qsrc = s->QJ2;
+      *coef = 2.0;
+    }
+  } else {
+    static const double w[3][2] = {{0.5, 1.0}, {0.5, 2.0}, {1.0, 2.0}};
+    if (st < 3) *sa = {s->QJ, s->QJ2, s->Rs, w[st][0], w[st][1], 0, ox::EP_RK4_S};
+    else *sa = {s->QJ, s->QJ, s->Rs, C.hc.one_sixth, 0, 0, ox::EP_RK4_S4};
+    *Qsrc = st == 0 ? s->QJ : s->QJ2;
+    *coef = 1.0;
+  }
+}
+
+size_t solver_footprint(const ox_config& c) {
+  const size_t ns = (size_t)c.nx * 5 * c.ny * c.nz, nc = (size_t)c.nx * c.ny * c.nz;
+  const size_t pe = c.precision == OX_PREC_FP64 ? sizeof(double) : sizeof(float);
+  const size_t nprim = c.visc == OX_VISC_LES ? 10 : 8;
+  size_t b = 2 * ns * sizeof(double) + ns * pe + 2 * nprim * nc * pe;  // QJ, QJ2, Racc, two primitive sets
+  if (c.rk == 4) b += ns * sizeof(double);                             // Rs
+  b += 4096 * (2 * ox::DIAG_NQ + 1) * sizeof(double);                  // reduction partials
+  return b;
+}
 
 namespace {
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
-
-#define OX_HIP(call)                                                                    \
-  do {                                                                                  \
-    hipError_t e_ = (call);                                                             \
-    if (e_ != hipSuccess) return fail(OX_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-size_t state_elems(const ox::Geom& g) { return (size_t)g.nx * 5 * g.ny * g.nz; }
-size_t cell_elems(const ox::Geom& g) { return (size_t)g.nx * g.ny * g.nz; }
-
 int validate(const ox_config& c) {
-  if (c.abi_version != OX_ABI_VERSION) return fail(OX_EINVAL, "abi_version mismatch");
-  if (c.order != 2 && c.order != 4 && c.order != 6) return fail(OX_EINVAL, "order must be 2, 4 or 6");
-  if (c.rk != 3 && c.rk != 4) return fail(OX_EINVAL, "rk must be 3 or 4");
-  if (c.bc_x || c.bc_y || c.bc_z) return fail(OX_EUNSUPPORTED, "only periodic boundaries are built");
+  if (c.abi_version != OX_ABI_VERSION) return ox_fail(OX_EINVAL, "abi_version mismatch");
+  if (c.order != 2 && c.order != 4 && c.order != 6) return ox_fail(OX_EINVAL, "order must be 2, 4 or 6");
+  if (c.rk != 3 && c.rk != 4) return ox_fail(OX_EINVAL, "rk must be 3 or 4");
+  if (c.bc_x || c.bc_y || c.bc_z) return ox_fail(OX_EUNSUPPORTED, "only periodic boundaries are built");
   if (c.visc_stencil != OX_STENCIL_CENTRAL && c.visc != OX_VISC_EULER)
-    return fail(OX_EUNSUPPORTED, "ME4Base viscous stencil is not built yet");
-  if (c.scheme < OX_SCHEME_KEEP || c.scheme > OX_SCHEME_HYBRID) return fail(OX_EINVAL, "unknown scheme");
-  if (c.tvd < 0 || c.tvd > 2) return fail(OX_EINVAL, "tvd must be 0, 1 or 2");
-  if (c.slau_variant < 0 || c.slau_variant > 1) return fail(OX_EINVAL, "slau_variant must be 0 or 1");
-  if (c.precision != OX_PREC_FP64 && c.precision != OX_PREC_FP32_BULK) return fail(OX_EINVAL, "bad precision");
+    return ox_fail(OX_EUNSUPPORTED, "ME4Base viscous stencil is not built yet");
+  if (c.scheme < OX_SCHEME_KEEP || c.scheme > OX_SCHEME_HYBRID) return ox_fail(OX_EINVAL, "unknown scheme");
+  if (c.tvd < 0 || c.tvd > 2) return ox_fail(OX_EINVAL, "tvd must be 0, 1 or 2");
+  if (c.slau_variant < 0 || c.slau_variant > 1) return ox_fail(OX_EINVAL, "slau_variant must be 0 or 1");
+  if (c.precision != OX_PREC_FP64 && c.precision != OX_PREC_FP32_BULK) return ox_fail(OX_EINVAL, "bad precision");
   const int h = c.order / 2;
   if (c.nx < 2 * h + 2 * c.order || c.ny < 2 * h + 2 * c.order || c.nz < 2 * h + 2 * c.order)
-    return fail(OX_EINVAL, "grid too small for the stencil");
-  if ((double)c.nx * 5.0 * c.ny * c.nz >= 2147483647.0) return fail(OX_EINVAL, "grid too large for 32-bit indexing");
-  if (c.nz > 65535 || c.ny > 65535 * 4) return fail(OX_EINVAL, "grid extent exceeds launch limits");
+    return ox_fail(OX_EINVAL, "grid too small for the stencil");
+  if ((double)c.nx * 5.0 * c.ny * c.nz >= 2147483647.0) return ox_fail(OX_EINVAL, "grid too large for 32-bit indexing");
+  if (c.nz > 65535 || c.ny > 65535 * 4) return ox_fail(OX_EINVAL, "grid extent exceeds launch limits");
   if (!(c.dt > 0.0) || !(c.Lx > 0.0) || !(c.Ly > 0.0) || !(c.Lz > 0.0) || !(c.gamma > 1.0) || !(c.R > 0.0))
-    return fail(OX_EINVAL, "non-positive dt, box length, gamma-1 or R");
+    return ox_fail(OX_EINVAL, "non-positive dt, box length, gamma-1 or R");
   return OX_OK;
 }
 
@@ -178,32 +186,19 @@ void destroy(ox_solver* s) {
   for (hipEvent_t e : s->prof.pool) hipEventDestroy(e);
   if (s->ev0) hipEventDestroy(s->ev0);
   if (s->ev1) hipEventDestroy(s->ev1);
-  if (s->C.stream) hipStreamDestroy(s->C.stream);
+  if (s->own_stream) hipStreamDestroy(s->own_stream);
   delete s;
 }
 
 int run_rk_step(ox_solver* s) {
   ox::Core& C = s->C;
   const ox::Geom& g = C.g;
-  ox::StageArgs sa{};
-  if (C.cfg.rk == 3) {
-    // calc_time_dev.f90.fypp:86-100
-    sa = {s->QJ, s->QJ2, s->Rs, 0, 0, 0, ox::EP_RK3_S1};
-    s->stage(C, s->QJ, sa, 1.0 * g.a_yz, 1.0 * g.a_zx, 1.0 * g.a_xy);
-    sa = {s->QJ, s->QJ2, s->Rs, 0.75, 0.25, 1.0, ox::EP_RK3_S23};
-    s->stage(C, s->QJ2, sa, 0.25 * g.a_yz, 0.25 * g.a_zx, 0.25 * g.a_xy);
-    sa = {s->QJ2, s->QJ, s->Rs, 2.0, 1.0, C.hc.one_third, ox::EP_RK3_S23};
-    s->stage(C, s->QJ2, sa, 2.0 * g.a_yz, 2.0 * g.a_zx, 2.0 * g.a_xy);
-  } else {
-    // calc_time_dev.f90.fypp:268-285
-    sa = {s->QJ, s->QJ2, s->Rs, 0.5, 1.0, 0, ox::EP_RK4_S};
-    s->stage(C, s->QJ, sa, g.a_yz, g.a_zx, g.a_xy);
-    sa = {s->QJ, s->QJ2, s->Rs, 0.5, 2.0, 0, ox::EP_RK4_S};
-    s->stage(C, s->QJ2, sa, g.a_yz, g.a_zx, g.a_xy);
-    sa = {s->QJ, s->QJ2, s->Rs, 1.0, 2.0, 0, ox::EP_RK4_S};
-    s->stage(C, s->QJ2, sa, g.a_yz, g.a_zx, g.a_xy);
-    sa = {s->QJ, s->QJ, s->Rs, C.hc.one_sixth, 0, 0, ox::EP_RK4_S4};
-    s->stage(C, s->QJ2, sa, g.a_yz, g.a_zx, g.a_xy);
+  for (int st = 0; st < C.cfg.rk; ++st) {
+    const double* Qsrc;
+    ox::StageArgs sa;
+    double coef;
+    rk_stage(s, st, &Qsrc, &sa, &coef);
+    s->stage(C, Qsrc, sa, coef * g.a_yz, coef * g.a_zx, coef * g.a_xy);
   }
   return OX_OK;
 }
@@ -225,12 +220,12 @@ const char* ox_last_error(void) { return g_err.c_str(); }
 const char* ox_version(void) { return "ouxsbli-mi355x 1 gfx950"; }
 
 int ox_create(const ox_config* cfg, ox_solver** out) {
-  if (!cfg || !out) return fail(OX_EINVAL, "null argument");
+  if (!cfg || !out) return ox_fail(OX_EINVAL, "null argument");
   *out = nullptr;
   int rc = validate(*cfg);
   if (rc != OX_OK) return rc;
   ox_solver* s = new (std::nothrow) ox_solver();
-  if (!s) return fail(OX_ENOMEM, "host allocation failed");
+  if (!s) return ox_fail(OX_ENOMEM, "host allocation failed");
   ox::Core& C = s->C;
   C.cfg = *cfg;
   C.g = make_geom(*cfg);
@@ -245,31 +240,33 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   C.cur = 0;
   C.prim_for = nullptr;
   C.stream = nullptr;
+  C.lseg[0] = C.lseg[1] = ox::LSEG_DEFAULT;
   C.prof = false;
   C.mark = &prof_record;
   C.prof_state = &s->prof;
   if (C.vorder != 0 && C.vorder != 4 && C.vorder != 6) {
     delete s;
-    return fail(OX_EINVAL, "visc_order must be 4 or 6");
+    return ox_fail(OX_EINVAL, "visc_order must be 4 or 6");
   }
   if (C.vorder == 6 && cfg->order != 6) {
     delete s;
-    return fail(OX_EINVAL, "visc_order 6 needs order 6 (three ghost layers)");
+    return ox_fail(OX_EINVAL, "visc_order 6 needs order 6 (three ghost layers)");
   }
   s->stage = cfg->precision == OX_PREC_FP64 ? ox::find_stage_fp64(cfg->order, cfg->scheme, cfg->visc, C.vorder)
                                             : ox::find_stage_fp32(cfg->order, cfg->scheme, cfg->visc, C.vorder);
   if (!s->stage) {
     delete s;
-    return fail(OX_EUNSUPPORTED, "no kernel specialisation for this (order, scheme, visc, visc_order)");
+    return ox_fail(OX_EUNSUPPORTED, "no kernel specialisation for this (order, scheme, visc, visc_order)");
   }
   hipError_t e = hipSetDevice(cfg->device);
   if (e != hipSuccess) {
     delete s;
-    return fail(OX_EDEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return ox_fail(OX_EDEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
   }
   const size_t ns = state_elems(C.g), nc = cell_elems(C.g);
   const size_t pe = cfg->precision == OX_PREC_FP64 ? sizeof(double) : sizeof(float);
   bool ok = hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking) == hipSuccess;
+  s->own_stream = C.stream;
   ok = ok && hipMalloc(&s->QJ, ns * sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc(&s->QJ2, ns * sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc(&C.Racc, ns * pe) == hipSuccess;
@@ -282,7 +279,7 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   ok = ok && hipEventCreate(&s->ev0) == hipSuccess && hipEventCreate(&s->ev1) == hipSuccess;
   if (!ok) {
     destroy(s);
-    return fail(OX_ENOMEM, "device allocation failed");
+    return ox_fail(OX_ENOMEM, "device allocation failed");
   }
   // Zero everything once: ghost layers of QJ2 and the accumulators start defined.
   ok = hipMemsetAsync(s->QJ, 0, ns * sizeof(double), C.stream) == hipSuccess;
@@ -294,7 +291,7 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   ok = ok && hipStreamSynchronize(C.stream) == hipSuccess;
   if (!ok) {
     destroy(s);
-    return fail(OX_EDEVICE, "device initialisation failed");
+    return ox_fail(OX_EDEVICE, "device initialisation failed");
   }
   *out = s;
   return OX_OK;
@@ -308,7 +305,7 @@ void ox_destroy(ox_solver* s) {
 }
 
 int ox_set_qj(ox_solver* s, const double* QJ) {
-  if (!s || !QJ) return fail(OX_EINVAL, "null argument");
+  if (!s || !QJ) return ox_fail(OX_EINVAL, "null argument");
   OX_HIP(hipSetDevice(s->C.cfg.device));
   const size_t n = state_elems(s->C.g);
   OX_HIP(hipMemcpyAsync(s->QJ, QJ, n * sizeof(double), hipMemcpyHostToDevice, s->C.stream));
@@ -319,7 +316,7 @@ int ox_set_qj(ox_solver* s, const double* QJ) {
 }
 
 int ox_get_qj(ox_solver* s, double* QJ) {
-  if (!s || !QJ) return fail(OX_EINVAL, "null argument");
+  if (!s || !QJ) return ox_fail(OX_EINVAL, "null argument");
   OX_HIP(hipSetDevice(s->C.cfg.device));
   const size_t n = state_elems(s->C.g);
   OX_HIP(hipMemcpyAsync(QJ, s->QJ, n * sizeof(double), hipMemcpyDeviceToHost, s->C.stream));
@@ -328,7 +325,7 @@ int ox_get_qj(ox_solver* s, double* QJ) {
 }
 
 int ox_set_state(ox_solver* s, const double* Q) {
-  if (!s || !Q) return fail(OX_EINVAL, "null argument");
+  if (!s || !Q) return ox_fail(OX_EINVAL, "null argument");
   int rc = ox_set_qj(s, Q);
   if (rc != OX_OK) return rc;
   const long long n = (long long)state_elems(s->C.g);
@@ -339,7 +336,7 @@ int ox_set_state(ox_solver* s, const double* Q) {
 }
 
 int ox_get_state(ox_solver* s, double* Q) {
-  if (!s || !Q) return fail(OX_EINVAL, "null argument");
+  if (!s || !Q) return ox_fail(OX_EINVAL, "null argument");
   OX_HIP(hipSetDevice(s->C.cfg.device));
   const long long n = (long long)state_elems(s->C.g);
   // J * QJ into the scratch-free second state buffer is not allowed (RK state);
@@ -352,8 +349,8 @@ int ox_get_state(ox_solver* s, double* Q) {
 }
 
 int ox_step(ox_solver* s, int32_t nsteps) {
-  if (!s) return fail(OX_EINVAL, "null handle");
-  if (nsteps < 0) return fail(OX_EINVAL, "nsteps < 0");
+  if (!s) return ox_fail(OX_EINVAL, "null handle");
+  if (nsteps < 0) return ox_fail(OX_EINVAL, "nsteps < 0");
   OX_HIP(hipSetDevice(s->C.cfg.device));
   OX_HIP(hipEventRecord(s->ev0, s->C.stream));
   if (s->C.prof) {
@@ -395,7 +392,7 @@ int ox_step(ox_solver* s, int32_t nsteps) {
 }
 
 int ox_rhs(ox_solver* s, double* R) {
-  if (!s || !R) return fail(OX_EINVAL, "null argument");
+  if (!s || !R) return ox_fail(OX_EINVAL, "null argument");
   OX_HIP(hipSetDevice(s->C.cfg.device));
   const ox::Geom& g = s->C.g;
   const size_t n = state_elems(g);
@@ -412,7 +409,7 @@ int ox_rhs(ox_solver* s, double* R) {
 }
 
 int ox_diagnostics(ox_solver* s, ox_diag* out) {
-  if (!s || !out) return fail(OX_EINVAL, "null argument");
+  if (!s || !out) return ox_fail(OX_EINVAL, "null argument");
   OX_HIP(hipSetDevice(s->C.cfg.device));
   const ox::Geom& g = s->C.g;
   const long long n = (long long)g.ix * g.iy * g.iz;
@@ -449,7 +446,7 @@ int ox_diagnostics(ox_solver* s, ox_diag* out) {
 }
 
 int ox_check_state(ox_solver* s, int64_t* bad_index, int64_t* count) {
-  if (!s) return fail(OX_EINVAL, "null handle");
+  if (!s) return ox_fail(OX_EINVAL, "null handle");
   OX_HIP(hipSetDevice(s->C.cfg.device));
   const ox::Geom& g = s->C.g;
   const long long n = (long long)g.ix * g.iy * g.iz;
@@ -469,20 +466,20 @@ int ox_check_state(ox_solver* s, int64_t* bad_index, int64_t* count) {
   }
   if (bad_index) *bad_index = first;
   if (count) *count = cnt;
-  if (cnt > 0) return fail(OX_ESTATE, "non-physical state: " + std::to_string(cnt) + " bad cells, first at linear index " +
+  if (cnt > 0) return ox_fail(OX_ESTATE, "non-physical state: " + std::to_string(cnt) + " bad cells, first at linear index " +
                                           std::to_string(first));
   return OX_OK;
 }
 
 int ox_synchronize(ox_solver* s) {
-  if (!s) return fail(OX_EINVAL, "null handle");
+  if (!s) return ox_fail(OX_EINVAL, "null handle");
   OX_HIP(hipSetDevice(s->C.cfg.device));
   OX_HIP(hipStreamSynchronize(s->C.stream));
   return OX_OK;
 }
 
 int ox_set_profiling(ox_solver* s, int32_t enable) {
-  if (!s) return fail(OX_EINVAL, "null handle");
+  if (!s) return ox_fail(OX_EINVAL, "null handle");
   s->C.prof = enable != 0;
   s->prof.acc_names.clear();
   s->prof.acc_ms.clear();
@@ -492,7 +489,7 @@ int ox_set_profiling(ox_solver* s, int32_t enable) {
 }
 
 int ox_kernel_times(ox_solver* s, ox_kernel_time* out, int32_t cap, int32_t* n) {
-  if (!s || !n) return fail(OX_EINVAL, "null argument");
+  if (!s || !n) return ox_fail(OX_EINVAL, "null argument");
   const ProfState& P = s->prof;
   *n = (int32_t)P.acc_names.size();
   for (int32_t a = 0; a < *n && a < cap && out; ++a) {
@@ -505,7 +502,7 @@ int ox_kernel_times(ox_solver* s, ox_kernel_time* out, int32_t cap, int32_t* n) 
 }
 
 int ox_last_step_ms(ox_solver* s, double* ms) {
-  if (!s || !ms) return fail(OX_EINVAL, "null argument");
+  if (!s || !ms) return ox_fail(OX_EINVAL, "null argument");
   *ms = s->last_ms;
   return OX_OK;
 }

@@ -27,6 +27,7 @@ struct Core {
   // (FP64 in the plain form; FP32 in the FP32-bulk form, where it carries
   // FP32 flux differences and the RK combination itself stays FP64)
   void* Racc;
+  int lseg[2];       // segment lengths of the y and z sweeps (tuned per case)
   hipStream_t stream;
   // profiling: when `prof` is set, mark() records an event after a kernel
   bool prof;
@@ -45,8 +46,52 @@ inline void prof_mark(Core& C, const char* name) {
 // cx, cy, cz are the calc_R coefficients (coef * dt * face area).
 using StageFn = void (*)(Core& C, const double* Qsrc, const StageArgs& sa, double cx, double cy, double cz);
 
+// Key of a kernel specialisation (ORDER, SCHEME, VISC, VISC_ORDER).
+inline int stage_key(int order, int scheme, int visc, int vorder) {
+  return order * 1000 + scheme * 100 + visc * 10 + (visc ? vorder : 0);
+}
+
 StageFn find_stage_fp64(int order, int scheme, int visc, int vorder);
 StageFn find_stage_fp32(int order, int scheme, int visc, int vorder);
+
+// After a stage whose epilogue wrote the primitives of sa.Qout into the
+// other primitive set, that set becomes current.
+inline void finish_stage(Core& C, const StageArgs& sa) {
+  if (sa.ep != EP_RHS) {
+    C.cur = 1 - C.cur;
+    C.prim_for = sa.Qout;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched stages.  The cases of one group share a kernel specialisation; one
+// stage of the whole group is three persistent launches (x, y, z sweeps)
+// over the concatenated tiles of its cases.
+// ---------------------------------------------------------------------------
+struct TileMap;
+
+struct BatchGroupLaunch {
+  hipStream_t stream;
+  int ncase;
+  const void* args[3];     // device arrays of the per-case sweep descriptors (x, y, z)
+  const TileMap* map[3];   // device tile maps (x, y, z)
+  int ntiles[3];
+  int grid[3];             // persistent grid sizes
+  int ep;                  // RK epilogue of the stage
+};
+
+struct BatchOps {
+  size_t args_bytes;  // size of one sweep descriptor
+  int xt;             // interior cells per x tile
+  // k_prim when the primitives are stale; the LES viscosity of the stage
+  void (*prepare)(Core& C, const double* Qsrc);
+  // the three sweep descriptors (x, y, z) of one case for one stage
+  void (*make_args)(Core& C, const StageArgs& sa, double cx, double cy, double cz, void* out3);
+  void (*launch)(const BatchGroupLaunch& L);
+};
+
+bool find_batch_fp64(int order, int scheme, int visc, int vorder, BatchOps* ops);
+bool find_batch_fp32(int order, int scheme, int visc, int vorder, BatchOps* ops);
 
 void launch_ghost_fill(Core& C, double* Q);
 

@@ -389,6 +389,42 @@ __device__ __forceinline__ void face_flux(const PhysC<R>& c, int tvd, int varian
 }
 
 // ---------------------------------------------------------------------------
+// Launch descriptors.  SweepArgs carries everything one sweep of one case
+// needs; the single-case kernels take it by value, the batched kernels read
+// an array of them (one per case) and map their blocks onto the cases' tiles
+// through a TileMap (first global tile index and tile counts per case).
+// ---------------------------------------------------------------------------
+template <class R, class S> struct SweepArgs {
+  Prim<S> P;        // primitives read by the stage
+  Prim<S> Pn;       // primitives of the new state (written by the z epilogue)
+  S* Racc;          // residual accumulator
+  Geom g;
+  PhysC<R> c;
+  PhysC<double> c64;
+  StageArgs sa;
+  double cd;        // c_x, c_y or c_z of calc_R (stage coefficient * dt * face area)
+  int tvd, variant;
+  int lseg;         // cells per thread segment in the y / z sweeps
+};
+
+struct TileMap {
+  int start;     // first global tile of the case
+  int nbx, nby;  // tiles along the first two launch dimensions
+  int pad;
+};
+
+// Case owning global tile t (map[].start ascending, map[0].start == 0).
+__device__ __forceinline__ int find_case(const TileMap* __restrict__ map, int ncase, int t) {
+  int lo = 0, hi = ncase - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (map[mid].start <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------
 // x sweep.  One wavefront per grid row: the tile of a row is exactly 64 cells
 // wide (XT = 64 - SW interior cells plus the stencil halo), so every lane
 // loads one cell, and lanes 0..XT compute the XT+1 faces the interior cells
@@ -462,9 +498,17 @@ __device__ __forceinline__ R ducros_x(const Geom& g, const DucrosPart<R>& d, R u
 // the schemes that read the Ducros sensor the sweep also computes it for its
 // row and stores it for the y and z sweeps.  The LDS tile is
 // structure-of-arrays so that lane-consecutive reads are bank-conflict free.
+template <int VISC> constexpr int xsweep_nf() { return VISC == 2 ? 14 : 12; }  // rho,u,v,w,p,T,mu,s,d1..d4[,mt,H]
+
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
-__global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, S* __restrict__ Racc, Geom g, PhysC<R> c, int tvd,
-                                                 int variant, double cx) {
+__device__ __forceinline__ void sweep_x_tile(const SweepArgs<R, S>& a, int bx, int by, int bz,
+                                             R (&tile)[xsweep_nf<VISC>()][XROWS][64]) {
+  const Prim<S>& P = a.P;
+  S* __restrict__ Racc = a.Racc;
+  const Geom& g = a.g;
+  const PhysC<R>& c = a.c;
+  const int tvd = a.tvd, variant = a.variant;
+  const double cx = a.cd;
   using St = Sten<ORDER, VISC, VO>;
   using CD = CellData<R, 0, ORDER, SCHEME, VISC, VO>;
   using RC = RawCell<S, VISC, VO>;
@@ -477,12 +521,11 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, S* __restrict__ Racc
   // index; with more layers every sensor the faces read belongs to a cell in
   // 1..nx-2 whose x neighbours are in the row tile.
   constexpr bool SPLIT_SENS = SENS && ORDER > 2;
-  constexpr int NF = VISC == 2 ? 14 : 12;  // rho,u,v,w,p,T,mu,s,d1..d4[,mt,H]
-  __shared__ R tile[NF][XROWS][W];
+  constexpr int NF = xsweep_nf<VISC>();
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i0 = g.h + blockIdx.x * XT;
-  const int j = g.h + blockIdx.y * XROWS + ty;
-  const int k = g.h + blockIdx.z;
+  const int i0 = g.h + bx * XT;
+  const int j = g.h + by * XROWS + ty;
+  const int k = g.h + bz;
   const bool rowok = j < g.h + g.iy;
   const int m = tx;                // this lane's tile position
   const int i = i0 - St::HW + m;   // and its cell
@@ -582,6 +625,26 @@ __global__ __launch_bounds__(256) void k_sweep_x(Prim<S> P, S* __restrict__ Racc
   }
 }
 
+template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
+__global__ __launch_bounds__(256) void k_sweep_x(SweepArgs<R, S> a) {
+  __shared__ R tile[xsweep_nf<VISC>()][XROWS][64];
+  sweep_x_tile<R, S, ORDER, SCHEME, VISC, VO>(a, blockIdx.x, blockIdx.y, blockIdx.z, tile);
+}
+
+// Persistent batched x sweep: block b handles global tiles b, b + gridDim.x, ...
+template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
+__global__ __launch_bounds__(256) void k_sweep_x_batch(const SweepArgs<R, S>* __restrict__ cases,
+                                                       const TileMap* __restrict__ map, int ncase, int ntiles) {
+  __shared__ R tile[xsweep_nf<VISC>()][XROWS][64];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int ci = find_case(map, ncase, t);
+    const int local = t - map[ci].start, nbx = map[ci].nbx, nby = map[ci].nby;
+    sweep_x_tile<R, S, ORDER, SCHEME, VISC, VO>(cases[ci], local % nbx, (local / nbx) % nby, local / (nbx * nby),
+                                                tile);
+    __syncthreads();  // the next tile overwrites the LDS rows
+  }
+}
+
 // The residual of one cell and, for the z sweep, the state values its RK
 // epilogue reads (A = Qin, B = Qout, C = Rs), per conserved variable.
 template <int DIR, int EP> struct CellState {
@@ -615,21 +678,28 @@ __device__ __forceinline__ void fetch_state(const S* __restrict__ Racc, const St
 // R += c_y*(F_right - F_left); DIR=2 adds c_z*(G_right - G_left) and applies
 // the stage epilogue.
 // ---------------------------------------------------------------------------
-constexpr int LSEG = 64;
+constexpr int LSEG_DEFAULT = 64;
 
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO, int DIR, int EP>
-__global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, S* __restrict__ Racc, Geom g, PhysC<R> c,
-                                                    int tvd, int variant, double cd, StageArgs sa, Prim<S> Pn,
-                                                    PhysC<double> c64) {
+__device__ __forceinline__ void sweep_line_tile(const SweepArgs<R, S>& a, int bx, int by, int bz) {
+  const Prim<S>& P = a.P;
+  const Prim<S>& Pn = a.Pn;
+  S* __restrict__ Racc = a.Racc;
+  const Geom& g = a.g;
+  const PhysC<R>& c = a.c;
+  const PhysC<double>& c64 = a.c64;
+  const StageArgs& sa = a.sa;
+  const int tvd = a.tvd, variant = a.variant;
+  const double cd = a.cd;
   using St = Sten<ORDER, VISC, VO>;
   using CD = CellData<R, DIR, ORDER, SCHEME, VISC, VO>;
-  const int i = g.h + blockIdx.x * 64 + threadIdx.x;
-  const int t = g.h + blockIdx.y * 4 + threadIdx.y;  // k for DIR 1, j for DIR 2
-  const int s0 = g.h + blockIdx.z * LSEG;
+  const int i = g.h + bx * 64 + threadIdx.x;
+  const int t = g.h + by * 4 + threadIdx.y;  // k for DIR 1, j for DIR 2
+  const int s0 = g.h + bz * a.lseg;
   const int nt = DIR == 1 ? g.iz : g.iy;
   const int nd = DIR == 1 ? g.iy : g.iz;
   if (i >= g.h + g.ix || t >= g.h + nt) return;
-  const int s_end = min(s0 + LSEG, g.h + nd);
+  const int s_end = min(s0 + a.lseg, g.h + nd);
   const int stride = DIR == 1 ? g.nx : g.nx * g.ny;
   const int mbase = DIR == 1 ? cidx(g, i, 0, t) : cidx(g, i, t, 0);
   const int qbase = DIR == 1 ? qidx(g, i, 0, 0, t) : qidx(g, i, 0, t, 0);
@@ -735,6 +805,23 @@ __global__ __launch_bounds__(256) void k_sweep_line(Prim<S> P, S* __restrict__ R
 #pragma unroll
     for (int m = 0; m < St::SW - 1; ++m) win[m] = win[m + 1];
     cs = ncs;
+  }
+}
+
+template <class R, class S, int ORDER, int SCHEME, int VISC, int VO, int DIR, int EP>
+__global__ __launch_bounds__(256) void k_sweep_line(SweepArgs<R, S> a) {
+  sweep_line_tile<R, S, ORDER, SCHEME, VISC, VO, DIR, EP>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Persistent batched y / z sweep over the tiles of several cases.
+template <class R, class S, int ORDER, int SCHEME, int VISC, int VO, int DIR, int EP>
+__global__ __launch_bounds__(256) void k_sweep_line_batch(const SweepArgs<R, S>* __restrict__ cases,
+                                                          const TileMap* __restrict__ map, int ncase, int ntiles) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int ci = find_case(map, ncase, t);
+    const int local = t - map[ci].start, nbx = map[ci].nbx, nby = map[ci].nby;
+    sweep_line_tile<R, S, ORDER, SCHEME, VISC, VO, DIR, EP>(cases[ci], local % nbx, (local / nbx) % nby,
+                                                            local / (nbx * nby));
   }
 }
 

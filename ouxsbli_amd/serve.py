@@ -1,0 +1,151 @@
+"""Serving loop: a stream of solve requests into the batch engine.
+
+Requests arrive at their own times (a seeded Poisson process in the
+benchmark).  Arrived requests queue first-in first-out; the head of the
+queue is admitted as soon as the engine has a free slot and enough free HBM
+budget for its footprint (``Batch.admit`` refuses otherwise), later requests
+wait behind it.  Every loop iteration advances all resident cases by one RK
+step together (continuous batching: a case joins at the next step after its
+admission and leaves the step after its last), then retires finished cases.
+Latency of a request = completion time - arrival time.  The reference has
+no serving path (one case per ``mpirun``, ouxsbli/case.py:180-201).
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+
+from .batch import Batch, Busy
+from .cases import CaseSpec, initial_state, khi_spec
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    spec: CaseSpec
+    nsteps: int
+    arrival: float                       # seconds after the stream starts
+    Q0: Optional[np.ndarray] = None      # initial physical state (the request payload)
+    admitted: Optional[float] = None
+    finished: Optional[float] = None
+    result: Optional[np.ndarray] = None
+
+    @property
+    def cell_stages(self) -> float:
+        return float(self.spec.interior_cells) * self.spec.stages * self.nsteps
+
+
+def khi_stream(n_requests: int, rate_hz: float = 20.0, seed: int = 20261207, n_min: int = 64, n_max: int = 256,
+               multiple: int = 16, nsteps: int = 200, precision: str = "fp32_bulk",
+               with_states: bool = True) -> List[Request]:
+    """The Kelvin-Helmholtz request stream: exponential inter-arrival times
+    (mean 1/rate_hz), interior size n uniform over the multiples of
+    ``multiple`` in [n_min, n_max], scheme KEEP or SLAU and ORDER 4 or 6 at
+    random, ``nsteps`` RK3 steps each, seeded."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for r in range(n_requests):
+        t += float(rng.exponential(1.0 / rate_hz))
+        n = int(rng.integers(n_min // multiple, n_max // multiple + 1)) * multiple
+        scheme = str(rng.choice(["KEEP", "SLAU"]))
+        order = int(rng.choice([4, 6]))
+        spec = khi_spec(n, order, scheme, precision, seed=seed + 1 + r)
+        reqs.append(Request(r, spec, nsteps, t, initial_state(spec) if with_states else None))
+    return reqs
+
+
+@dataclasses.dataclass
+class ServeReport:
+    requests: List[Request]
+    wall_s: float
+    batch_stats: Dict
+    steps: int
+
+    def latencies(self) -> np.ndarray:
+        return np.array([r.finished - r.arrival for r in self.requests])
+
+    def summary(self) -> Dict:
+        lat = self.latencies()
+        cs = sum(r.cell_stages for r in self.requests)
+        return {
+            "requests": len(self.requests),
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()),
+            "max_latency_s": float(lat.max()),
+            "wall_s": self.wall_s,
+            "Mcell_updates_per_s_stage": cs / self.wall_s / 1e6,
+            "device_busy_Mcell_updates_per_s_stage": cs / (self.batch_stats["device_ms"] * 1e-3) / 1e6
+            if self.batch_stats["device_ms"] > 0 else None,
+            "batch_steps": self.steps,
+            "graph_captures": self.batch_stats["graph_captures"],
+            "graph_replays": self.batch_stats["graph_replays"],
+        }
+
+
+def serve(requests: List[Request], hbm_budget: int, max_cases: int = 64, use_graph: bool = True,
+          autotune: bool = True, pretune: bool = True, keep_results: bool = True,
+          clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] = time.sleep,
+          batch: Optional[Batch] = None) -> ServeReport:
+    """Runs the stream to completion and returns the per-request timings.
+
+    ``pretune`` fills the engine's segment-length cache for every distinct
+    (specialisation, grid) of the stream before the clock starts, the way a
+    server keeps tuned plans for the shapes it serves."""
+    own = batch is None
+    if own:
+        batch = Batch(hbm_budget, max_cases=max_cases, use_graph=use_graph, autotune=autotune)
+    try:
+        for r in requests:
+            if r.Q0 is None:
+                r.Q0 = initial_state(r.spec)
+        if pretune and autotune:
+            seen = set()
+            for r in requests:
+                s = r.spec
+                key = (s.precision, s.scheme, s.order, s.nx, s.ny, s.nz)
+                if key not in seen:
+                    seen.add(key)
+                    batch.tune(s, r.Q0)
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        waiting: collections.deque = collections.deque()
+        by_id: Dict[int, Request] = {}
+        active = 0
+        steps = 0
+        t0 = clock()
+        while pending or waiting or active:
+            now = clock() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            while waiting:
+                r = waiting[0]
+                try:
+                    cid = batch.admit(r.spec, r.Q0, r.nsteps)
+                except Busy:
+                    break
+                r.admitted = clock() - t0
+                by_id[cid] = r
+                waiting.popleft()
+                active += 1
+            if active == 0:
+                if pending:
+                    sleep(max(0.0, pending[0].arrival - (clock() - t0)))
+                continue
+            active = batch.step()
+            steps += 1
+            for cid in batch.done():
+                r = by_id.pop(cid)
+                Q = batch.retire(cid)
+                r.finished = clock() - t0
+                if keep_results:
+                    r.result = Q
+        wall = clock() - t0
+        return ServeReport(requests, wall, batch.stats(), steps)
+    finally:
+        if own:
+            batch.close()

@@ -112,5 +112,12 @@ class Solver:
         _native.check(self._lib.ox_kernel_times(self._h, arr, cap, ctypes.byref(n)))
         return {arr[a].name.decode(): (arr[a].total_ms, arr[a].launches) for a in range(min(n.value, cap))}
 
+    def autotune(self):
+        """Times candidate y/z sweep segment lengths on this grid and keeps
+        the fastest; returns (lseg_y, lseg_z).  The state is unchanged."""
+        ly, lz = ctypes.c_int32(), ctypes.c_int32()
+        _native.check(self._lib.ox_autotune(self._h, ctypes.byref(ly), ctypes.byref(lz)))
+        return ly.value, lz.value
+
     def synchronize(self) -> None:
         _native.check(self._lib.ox_synchronize(self._h))

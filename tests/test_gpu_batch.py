@@ -1,0 +1,128 @@
+"""Batch engine and serving loop on the GPU.
+
+The bar: a case advanced in a batch, among other cases that join and leave
+at other steps, ends bit-identical to the same case advanced alone with
+ox_step (same kernels, same data, per-case descriptors), in both the FP64
+and the FP32-bulk forms; and an FP64 batched case is bit-identical to the
+CPU oracle.
+"""
+import numpy as np
+import pytest
+
+from ouxsbli_amd import cases, serve
+from ouxsbli_amd.batch import Batch, Busy, footprint
+from ouxsbli_amd.solver import Solver
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+GIB = 1 << 30
+
+
+def _alone(spec, Q, n):
+    with Solver(spec) as s:
+        s.set_state(Q)
+        s.step(n)
+        return s.get_state()
+
+
+def _mix_specs(precision):
+    # (spec, nsteps, admitted before batch step #)
+    return [
+        (cases.khi_spec(24, 4, "KEEP", precision, seed=1), 7, 0),
+        (cases.khi_spec(32, 6, "SLAU", precision, seed=2), 5, 0),
+        (cases.khi_spec(16, 4, "SLAU", precision, seed=3), 4, 2),
+        (cases.khi_spec(24, 6, "KEEP", precision, seed=4), 3, 3),
+        (cases.tgv_spec(20, 6, Ma=1.25, cfl=0.3, cfl_mode="acoustic", scheme="SLAU", visc="NS", visc_order=6, rk=3,
+                        tvd="hybrid", precision=precision), 4, 1),
+        (cases.tgv_spec(16, 4, Ma=0.5, cfl=0.3, cfl_mode="acoustic", scheme="Roe", visc="NS", visc_order=4, rk=4,
+                        tvd="tvd", precision=precision), 5, 1),
+    ]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_bulk"])
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_alone_equals_in_mix(precision, use_graph):
+    mix = _mix_specs(precision)
+    states = [cases.initial_state(s) for s, _, _ in mix]
+    ref = [_alone(s, Q, n) for (s, n, _), Q in zip(mix, states)]
+    out = {}
+    with Batch(hbm_budget=8 * GIB, max_cases=16, use_graph=use_graph, autotune=True) as b:
+        ids = {}
+        step = 0
+        while len(out) < len(mix):
+            for k, (s, n, at) in enumerate(mix):
+                if at == step and k not in ids.values():
+                    ids[b.admit(s, states[k], n)] = k
+            b.step()
+            step += 1
+            for cid in b.done():
+                out[ids[cid]] = b.retire(cid)
+            assert step < 50
+        st = b.stats()
+    for k in range(len(mix)):
+        np.testing.assert_array_equal(out[k], ref[k], err_msg=f"case {k}")
+    assert st["steps"] == step
+    if use_graph:
+        assert st["graph_replays"] == step and st["graph_captures"] >= 1
+
+
+def test_batched_fp64_equals_oracle():
+    spec = cases.tgv_spec(20, 6, Ma=1.25, cfl=0.3, cfl_mode="acoustic", scheme="SLAU", visc="NS", visc_order=6, rk=3,
+                          tvd="hybrid")
+    other = cases.khi_spec(24, 4, "KEEP", "fp64", seed=9)
+    Q = cases.initial_state(spec)
+    o = Oracle(spec)
+    o.set_state(Q)
+    o.step(4)
+    with Batch(hbm_budget=4 * GIB, max_cases=4) as b:
+        a = b.admit(spec, Q, 4)
+        b.admit(other, cases.initial_state(other), 6)
+        while a not in b.done():
+            b.step()
+        Qg = b.retire(a)
+    np.testing.assert_array_equal(Qg, o.get_state())
+
+
+def test_admission_by_footprint_and_slots():
+    s1 = cases.khi_spec(32, 4, "KEEP", "fp64", seed=1)
+    s2 = cases.khi_spec(24, 6, "SLAU", "fp64", seed=2)
+    f1, f2 = footprint(s1), footprint(s2)
+    assert f1 > 0 and f2 > 0
+    Q1, Q2 = cases.initial_state(s1), cases.initial_state(s2)
+    with Batch(hbm_budget=f1 + f2 - 1, max_cases=4) as b:
+        a = b.admit(s1, Q1, 2)
+        with pytest.raises(Busy):
+            b.admit(s2, Q2, 2)
+        assert b.stats()["bytes_in_use"] == f1
+        while b.step():
+            pass
+        b.retire(a)
+        assert b.stats()["bytes_in_use"] == 0
+        b.admit(s2, Q2, 2)  # fits now
+    with Batch(hbm_budget=8 * GIB, max_cases=1) as b:
+        b.admit(s1, Q1, 1)
+        with pytest.raises(Busy):
+            b.admit(s2, Q2, 1)
+
+
+def test_retire_errors():
+    s = cases.khi_spec(16, 4, "KEEP", "fp64", seed=1)
+    with Batch(hbm_budget=GIB, max_cases=2) as b:
+        a = b.admit(s, cases.initial_state(s), 3)
+        with pytest.raises(RuntimeError):
+            b.retire(a)  # still stepping
+        with pytest.raises(RuntimeError):
+            b._shapes[12345] = s.state_shape()
+            b.retire(12345)
+
+
+def test_serve_stream_results_equal_alone():
+    reqs = serve.khi_stream(6, rate_hz=200.0, seed=11, n_min=16, n_max=48, nsteps=5, precision="fp32_bulk")
+    rep = serve.serve(reqs, hbm_budget=4 * GIB, max_cases=3)
+    for r in rep.requests:
+        assert r.finished is not None and r.finished > r.arrival
+        np.testing.assert_array_equal(r.result, _alone(r.spec, r.Q0, r.nsteps), err_msg=f"request {r.rid}")
+    s = rep.summary()
+    assert s["requests"] == 6 and s["p99_latency_s"] >= s["p50_latency_s"] > 0
+    assert s["Mcell_updates_per_s_stage"] > 0

@@ -8,7 +8,10 @@ hybrid TVD (threshold 0.4), central-6 viscous terms, TVD-RK3, CFL 0.03
 stages).  `value` is the FP64 plain form; the FP32-bulk form of the same
 workload is measured beside it.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+A serving leg (the Kelvin-Helmholtz request stream through the batch engine)
+reports p50/p99 solve latency beside throughput.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu] [--serve-requests N]
 """
 import argparse
 import ctypes
@@ -32,21 +35,26 @@ def _opcounts():
 
 def kernel_model(spec, ops):
     """Compulsory HBM bytes and algorithmic FP ops per launch of each kernel of
-    the stage pipeline (FP64 element = 8 B).  Bytes count every array element a
-    kernel must read or write once; ops come from ouxsbli_amd/flops/opcount.json."""
+    the stage pipeline.  Bytes count every array element a kernel must read or
+    write once: primitives and the residual accumulator in the form's element
+    type (8 B FP64, 4 B FP32-bulk), the state in FP64.  Ops come from
+    ouxsbli_amd/flops/opcount.json."""
     padded = spec.nx * spec.ny * spec.nz
     cells = spec.interior_cells
     e = 8 if spec.precision == "fp64" else 4
-    nprim = 8  # rho,u,v,w,p,T,mu,sensor
     face = ops["conv_face_smooth"] + 2 + ops["visc_face"] + 4 * ops["cross_deriv"]
     m = {
+        # stale primitives only (after a host upload)
         "prim": {"bytes": padded * (40 + 7 * e), "ops": padded * 20},
-        "sensor": {"bytes": padded * (3 * e + e), "ops": padded * 39},
-        "sweep_x": {"bytes": cells * (nprim * e + 40), "ops": cells * (face + 10)},
-        "sweep_y": {"bytes": cells * (nprim * e + 80), "ops": cells * (face + 15)},
-        # average over the three RK3 stages: stage 1 reads Qin, writes Qout;
-        # stages 2-3 also read Qout
-        "sweep_z_rk": {"bytes": cells * (nprim * e + 40 + 40 + 40 + 40 * 2 / 3), "ops": cells * (face + 15 + 4 * 5)},
+        # reads rho,u,v,w,p,T,mu; writes the sensor and the residual
+        "sweep_x": {"bytes": cells * (7 * e + e + 5 * e), "ops": cells * (face + 10)},
+        # reads the 8 primitives; reads and writes the residual
+        "sweep_y": {"bytes": cells * (8 * e + 10 * e), "ops": cells * (face + 15)},
+        # reads the 8 primitives and the residual; RK3 average of the state
+        # traffic (stage 1: Qin read, Qout write; stages 2-3: also Qout read);
+        # writes the 7 primitives of the new state
+        "sweep_z_rk": {"bytes": cells * (8 * e + 5 * e + 40 + 40 + 40 * 2 / 3 + 7 * e),
+                       "ops": cells * (face + 15 + 4 * 5 + 20)},
         "ghost_fill": {"bytes": (padded - cells) * 80, "ops": 0},
     }
     return m
@@ -58,10 +66,10 @@ def kernel_model(spec, ops):
 # after a kernel change.
 MEASURED_TRAFFIC_256 = {
     "prim": 1957e6,
-    "sweep_x": 2936e6,
-    "sweep_y": 3504e6,
-    "sweep_z_rk": (4374e6 + 2 * 5214e6) / 3,
-    "ghost_fill": 176e6,
+    "sweep_x": 2950e6,
+    "sweep_y": 3399e6,
+    "sweep_z_rk": (5426e6 + 2 * 6268e6) / 3,
+    "ghost_fill": 175e6,
 }
 
 
@@ -86,6 +94,30 @@ def cpu_baseline(spec_small, seconds=15.0):
     return {"value": rate, "unit": "Mcell-updates/s per RK stage", "cores": cores, "kind": "port",
             "sample": f"oracle (oracle/liboxoracle.so, FP64 OpenMP) on the same scheme at "
                       f"{spec_small.interior[0]}^3 interior, 1 warm-up + {n} timed RK3 steps ({el:.1f} s)"}
+
+
+SERVE_BUDGET = 64 << 30
+SERVE_SLOTS = 16
+
+
+def serving_leg(n_requests, fp64_value):
+    """Kelvin-Helmholtz serving stream through the batch engine (FP32-bulk):
+    seeded Poisson arrivals at 20 req/s, interior grids 64^3..256^3 (multiples
+    of 16), KEEP or SLAU at ORDER 4 or 6, Euler, RK3, 200 steps each; FIFO
+    admission by HBM footprint (64 GiB budget) and 16 slots."""
+    from ouxsbli_amd import serve
+    reqs = serve.khi_stream(n_requests, rate_hz=20.0, nsteps=200, precision="fp32_bulk")
+    rep = serve.serve(reqs, hbm_budget=SERVE_BUDGET, max_cases=SERVE_SLOTS, keep_results=False)
+    s = rep.summary()
+    out = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()}
+    out["workload"] = ("3-D Kelvin-Helmholtz stream: double tanh shear layer d=L/28, Ma 0.5, 1e-3 U seeded noise, "
+                       f"{n_requests} requests, Poisson 20 req/s, grids 64^3-256^3, KEEP/SLAU ORDER 4/6, Euler RK3, "
+                       "200 steps each, FP32-bulk, batched per stage with HIP graph replay")
+    out["hbm_budget_GiB"] = SERVE_BUDGET >> 30
+    out["slots"] = SERVE_SLOTS
+    busy = s["device_busy_Mcell_updates_per_s_stage"]
+    out["device_busy_vs_fp64_plain"] = round(busy / fp64_value, 3) if busy else None
+    return out
 
 
 def run_form(spec, steps, warmup, profile):
@@ -116,6 +148,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests of the Kelvin-Helmholtz serving stream (0: skip the serving leg)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no multi-GPU path; "
@@ -183,6 +217,8 @@ def main():
         "kernels": kernels,
         "state_ok": bool(r64["ok"] and r32["ok"]),
     }
+    if args.serve_requests > 0:
+        line["serving"] = serving_leg(args.serve_requests, value)
     if not args.no_cpu:
         small = cases.config2_spec(64, "fp64")
         line["cpu_baseline"] = cpu_baseline(small, args.cpu_seconds)

@@ -233,24 +233,29 @@ def khi_spec(n: int, order: int, scheme: str, precision: str = "fp64", seed: int
 def khi_state(spec: CaseSpec) -> np.ndarray:
     """Double tanh shear layer u(y) = U/2 * (tanh((y-1/4)/d) - tanh((y-3/4)/d) - 1),
     d = L/28, plus seeded noise of amplitude 1e-3 U in u, v, w (uniform in
-    [-1, 1]); uniform density and pressure."""
-    x = cell_centres(spec.nx, spec.Lx, spec.order)
+    [-1, 1], drawn in that order over the whole (z, y, x) array); uniform
+    density and pressure."""
     y = cell_centres(spec.ny, spec.Ly, spec.order)
-    z = cell_centres(spec.nz, spec.Lz, spec.order)
-    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
     U, d = spec.V0, spec.Ly / 28.0
     rng = np.random.default_rng(spec.seed)
-    shape = X.shape
-    u = 0.5 * U * (np.tanh((Y - 0.25 * spec.Ly) / d) - np.tanh((Y - 0.75 * spec.Ly) / d) - 1.0)
-    u = u + 1e-3 * U * rng.uniform(-1.0, 1.0, shape)
-    v = 1e-3 * U * rng.uniform(-1.0, 1.0, shape)
-    w = 1e-3 * U * rng.uniform(-1.0, 1.0, shape)
-    rho = np.full(shape, spec.RHO0)
-    p = np.full(shape, spec.p0)
-    Q = np.zeros(spec.state_shape(), dtype=np.float64)
+    shape = (spec.nz, spec.ny, spec.nx)
+    prof = 0.5 * U * (np.tanh((y - 0.25 * spec.Ly) / d) - np.tanh((y - 0.75 * spec.Ly) / d) - 1.0)
+    amp = 1e-3 * U
+    u = rng.uniform(-1.0, 1.0, shape)
+    u *= amp
+    u += prof[None, :, None]
+    v = rng.uniform(-1.0, 1.0, shape)
+    v *= amp
+    w = rng.uniform(-1.0, 1.0, shape)
+    w *= amp
+    rho, p = spec.RHO0, spec.p0
+    Q = np.empty(spec.state_shape(), dtype=np.float64)
     Q[:, :, 0, :] = rho
     Q[:, :, 1, :] = rho * u
     Q[:, :, 2, :] = rho * v
     Q[:, :, 3, :] = rho * w
-    Q[:, :, 4, :] = p / (spec.gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+    ke = u * u
+    ke += v * v
+    ke += w * w
+    Q[:, :, 4, :] = p / (spec.gamma - 1.0) + 0.5 * rho * ke
     return periodic_fill(Q, spec.order)

@@ -13,7 +13,9 @@ no serving path (one case per ``mpirun``, ouxsbli/case.py:180-201).
 from __future__ import annotations
 
 import collections
+import concurrent.futures
 import dataclasses
+import os
 import time
 from typing import Callable, Dict, List, Optional
 
@@ -55,7 +57,13 @@ def khi_stream(n_requests: int, rate_hz: float = 20.0, seed: int = 20261207, n_m
         scheme = str(rng.choice(["KEEP", "SLAU"]))
         order = int(rng.choice([4, 6]))
         spec = khi_spec(n, order, scheme, precision, seed=seed + 1 + r)
-        reqs.append(Request(r, spec, nsteps, t, initial_state(spec) if with_states else None))
+        reqs.append(Request(r, spec, nsteps, t))
+    if with_states:
+        # each request's state comes from its own seeded generator, so the
+        # threads do not change what is generated
+        with concurrent.futures.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+            for r, Q in zip(reqs, ex.map(lambda q: initial_state(q.spec), reqs)):
+                r.Q0 = Q
     return reqs
 
 
@@ -149,3 +157,32 @@ def serve(requests: List[Request], hbm_budget: int, max_cases: int = 64, use_gra
     finally:
         if own:
             batch.close()
+
+
+def main(argv=None) -> int:
+    """Runs a Kelvin-Helmholtz request stream and prints one JSON line with
+    the latency percentiles, throughput and engine statistics."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--requests", type=int, default=24)
+    ap.add_argument("--rate", type=float, default=20.0, help="mean arrivals per second")
+    ap.add_argument("--n-min", type=int, default=64)
+    ap.add_argument("--n-max", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--precision", default="fp32_bulk", choices=["fp64", "fp32_bulk"])
+    ap.add_argument("--budget-gib", type=float, default=64.0)
+    ap.add_argument("--slots", type=int, default=16)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--seed", type=int, default=20261207)
+    a = ap.parse_args(argv)
+    reqs = khi_stream(a.requests, rate_hz=a.rate, seed=a.seed, n_min=a.n_min, n_max=a.n_max, nsteps=a.steps,
+                      precision=a.precision)
+    rep = serve(reqs, hbm_budget=int(a.budget_gib * (1 << 30)), max_cases=a.slots, use_graph=not a.no_graph,
+                keep_results=False)
+    print(json.dumps({**rep.summary(), "engine": rep.batch_stats}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

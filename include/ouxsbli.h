@@ -133,9 +133,12 @@ int ox_get_state(ox_solver* s, double* Q);
 int ox_set_qj(ox_solver* s, const double* QJ);
 int ox_get_qj(ox_solver* s, double* QJ);
 
-/* Advances nsteps full RK steps (RK3: 3 stages, RK4: 4 stages) with the
- * periodic ghost fill after every stage.  Replaces the inner `t1` loop of
- * RungeKutta (3D_solver/src/calc_time_dev.f90.fypp:84-101 and :266-286). */
+/* Advances nsteps full RK steps (RK3: 3 stages, RK4: 4 stages).  Each stage
+ * keeps the primitives of the ghost cells equal to those of their periodic
+ * images, which is what the reference's per-stage set_bc achieves; the
+ * state's own ghost layers are filled once at the end of the call.  Replaces
+ * the inner `t1` loop of RungeKutta (3D_solver/src/calc_time_dev.f90.fypp:84-101
+ * and :266-286). */
 int ox_step(ox_solver* s, int32_t nsteps);
 
 /* One right-hand-side evaluation on the handle's current state: writes

@@ -12,20 +12,28 @@
 // source left to right; build with -ffp-contract=off so that no fused
 // multiply-add changes a rounding.
 //
-// Parity pins (tests/test_oracle_*.py):
-//   grid / Jacobian / pre_calc   closed forms (uniform periodic grid)
-//   primitives                   conserved -> primitive -> conserved round trip
-//   Ducros sensor                rotation -> 0, pure compression -> ~1
+// Parity pins (tests/test_oracle.py, tests/test_oracle_pins.py,
+// tests/test_golden.py):
+//   grid / Jacobian / pre_calc   TGV initial KE closed form
+//   primitives                   primitives rebuild the conserved state;
+//                                T = p/(R rho); Sutherland literals
+//   Ducros sensor                u(y),v(x) -> exactly 0; u(x),v(y),w(z) -> 1
 //   KEEP2/4/6                    F(q,q) = Euler flux, free stream -> RHS == 0,
 //                                periodic conservation, convergence order
-//   MUSCL (all TVD kinds)        exact on linear data, minmod bounds, brute force
+//                                (reference thresholds 3.6 / 5.7)
+//   MUSCL (all TVD kinds)        exact on linear data, minmod bounds
 //   wiggle detector              monotone -> 0, oscillating -> 1
-//   SLAU1 / HRSLAU2              F(q,q) = Euler flux, mirror symmetry
-//   central viscous 4/6          linear velocity -> exact constant stress,
-//                                quadratic T -> exact heat flux
-//   RK3 / RK4 steps              stability polynomials on dQ/dt = lambda Q
+//   SLAU1 / HRSLAU2 / Roe        F(q,q) = Euler flux, mirror symmetry,
+//                                supersonic upwinding; hybrid reduces to
+//                                KEEP / SLAU(HR=1)
+//   central viscous 4/6          discrete symbol on a shear wave (textbook
+//                                staggered weights), order on a shear wave
+//   RK3 / RK4 steps              Shu-Osher and classical tableaux composed
+//                                from the oracle's own right-hand sides;
+//                                time order
 //   periodic fill                ghost == periodic image, bitwise
-//   diagnostics                  TGV initial KE closed form, brute force sums
+//   LES (SMS)                    off for a single mode, on for a multi-mode
+//   everything above             golden digests (tests/golden/)
 #pragma once
 
 #include <algorithm>

@@ -1,0 +1,224 @@
+"""More pins of the FP64 CPU oracle: primitive recovery and the Sutherland
+literals, the Ducros sensor's closed forms, the viscous operator's discrete
+symbol on a shear wave, and the RK3/RK4 tableaux composed from the oracle's
+own right-hand sides.  Each expectation is derived independently of the
+oracle's code (textbook relations, exact discrete symbols of the stated
+stencils), so that a dropped term, a sign, an index or a wrong coefficient
+fails."""
+import math
+
+import numpy as np
+import pytest
+
+from ouxsbli_amd import cases
+from oracle_lib import Oracle
+
+
+def _interior(spec):
+    h = spec.order // 2
+    return (slice(h, spec.nz - h), slice(h, spec.ny - h), slice(None), slice(h, spec.nx - h))
+
+
+def _cells(spec):
+    h = spec.order // 2
+    return (slice(h, spec.nz - h), slice(h, spec.ny - h), slice(h, spec.nx - h))
+
+
+def _pack(spec, rho, u, v, w, p):
+    Q = np.zeros(spec.state_shape())
+    Q[:, :, 0, :] = rho
+    Q[:, :, 1, :] = rho * u
+    Q[:, :, 2, :] = rho * v
+    Q[:, :, 3, :] = rho * w
+    Q[:, :, 4, :] = p / (spec.gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+    return cases.periodic_fill(Q, spec.order)
+
+
+def _grid(spec):
+    x = cases.cell_centres(spec.nx, spec.Lx, spec.order)
+    y = cases.cell_centres(spec.ny, spec.Ly, spec.order)
+    z = cases.cell_centres(spec.nz, spec.Lz, spec.order)
+    return np.meshgrid(z, y, x, indexing="ij")
+
+
+def _spec(n=12, order=4, **kw):
+    base = dict(scheme="KEEP", visc="NS", visc_order=order, tvd="none", rk=3)
+    base.update(kw)
+    return cases.tgv_spec(n, order, Ma=0.3, cfl=0.3, cfl_mode="acoustic", **base)
+
+
+# ------------------------------------------------------------ primitives --
+def test_primitive_round_trip_and_sutherland_literals():
+    """calc_quantities_T_3D (src/calc_physical_quantities.f90:60-116): the
+    primitives rebuild the conserved state, T = p/(R rho), and mu follows
+    Sutherland's law with the literals of mod_constant.f90.fypp:26
+    (1.716e-5 Pa s at 273.2 K, S = 111 K)."""
+    spec = _spec(10, 4)
+    rng = np.random.default_rng(5)
+    Z, Y, X = _grid(spec)
+    rho = 1.2 * (1 + 0.2 * rng.uniform(-1, 1, X.shape))
+    u, v, w = (30 * rng.uniform(-1, 1, X.shape) for _ in range(3))
+    p = 1.0e5 * (1 + 0.3 * rng.uniform(-1, 1, X.shape))
+    Q = _pack(spec, rho, u, v, w, p)
+    o = Oracle(spec)
+    o.set_state(Q)
+    o.rhs()
+    Qp, T, mu, _ = o.primitives()
+    c = _cells(spec)
+    sl = _interior(spec)
+    r_, u_, v_, w_, p_ = (Qp[:, :, l, :][c] for l in range(5))
+    rebuilt = np.stack([r_, r_ * u_, r_ * v_, r_ * w_, p_ / (spec.gamma - 1) + 0.5 * r_ * (u_ * u_ + v_ * v_ + w_ * w_)],
+                       axis=2)
+    np.testing.assert_allclose(rebuilt, Q[sl], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(T[c], p_ / (spec.R * r_), rtol=1e-14)
+    Tc = T[c]
+    mu_ref = 1.716e-5 * (273.2 + 111.0) / 273.2 ** 1.5 * Tc ** 1.5 / (Tc + 111.0)
+    np.testing.assert_allclose(mu[c], mu_ref, rtol=1e-13)
+
+
+# ---------------------------------------------------------------- Ducros --
+def _sensor(spec, u, v, w):
+    Z, Y, X = _grid(spec)
+    one = np.ones_like(X)
+    Q = _pack(spec, 1.1 * one, u, v, w, 1.0e5 * one)
+    o = Oracle(spec)
+    o.set_state(Q)
+    o.rhs()
+    return o.primitives()[3][_cells(spec)]
+
+
+def test_ducros_zero_for_solenoidal_shear():
+    """calc_Ducros (3D_solver/src/calc_hybrid.f90:11-72): for u(y), v(x) the
+    discrete dilatation vanishes identically, so the sensor is exactly 0."""
+    spec = _spec(12, 4, scheme="SLAU", visc="Euler", visc_order=0, tvd="hybrid")
+    Z, Y, X = _grid(spec)
+    s = _sensor(spec, 20 * np.sin(Y), -20 * np.sin(X), 0 * X)
+    assert np.all(s == 0.0)
+
+
+def test_ducros_one_for_irrotational_compression():
+    """u(x), v(y), w(z): the discrete vorticity vanishes identically, so the
+    sensor is div^2/(div^2 + 1e-12) -> 1 wherever the dilatation is O(1),
+    and never exceeds 1."""
+    spec = _spec(12, 4, scheme="SLAU", visc="Euler", visc_order=0, tvd="hybrid")
+    Z, Y, X = _grid(spec)
+    s = _sensor(spec, 20 * np.sin(X), 20 * np.sin(Y), 20 * np.sin(Z))
+    div = 20 * (np.cos(X) + np.cos(Y) + np.cos(Z))[_cells(spec)]
+    assert np.all(s <= 1.0)
+    big = np.abs(div) > 1.0
+    assert big.sum() > 100
+    assert np.all(s[big] > 1 - 1e-9)
+
+
+# ------------------------------------------------- viscous shear symbol --
+# Textbook staggered first-derivative weights (face value from the cells at
+# offsets -3/2.. in units of dy): 4th order (1/24, -27/24), 6th order
+# (-3/640, 25/384, -75/64) -- antisymmetric.
+_D = {4: [1 / 24, -27 / 24, 27 / 24, -1 / 24],
+      6: [-3 / 640, 25 / 384, -75 / 64, 75 / 64, -25 / 384, 3 / 640]}
+
+
+@pytest.mark.parametrize("vo", [4, 6])
+def test_viscous_shear_wave_discrete_symbol(vo):
+    """u = U sin(k y), v = w = 0, uniform rho and p: every convective flux
+    difference vanishes, the continuity and y/z-momentum residuals are zero,
+    and the x-momentum residual is the exact discrete symbol of
+    -dt*A*d/dy(mu du/dy) for the central VO-point face derivative
+    (calc_visc_cent.f90.fypp diff4/diff6 with the flux difference of
+    calc_steps.f90:8-38): R_u(j) = -c_y mu s(k) (sin k y_{j+1/2} ... ) with
+    s(k) = sum_m w_m sin(k (m - VO/2 + 1/2) dy) / dy.  The reference's
+    truncated literal 0.065104167 (for 25/384) is within the 1e-8 tolerance;
+    a wrong weight, sign or index is not."""
+    order = 6 if vo == 6 else 4
+    n = 24
+    spec = cases.tgv_spec(n, order, Ma=0.3, cfl=0.3, cfl_mode="acoustic", scheme="KEEP", visc="NS",
+                          visc_order=vo, tvd="none", rk=3)
+    Z, Y, X = _grid(spec)
+    one = np.ones_like(X)
+    k = 3.0 * 2 * math.pi / spec.Ly
+    U = 15.0
+    rho0, p0 = 1.2, 1.0e5
+    Q = _pack(spec, rho0 * one, U * np.sin(k * Y), 0 * X, 0 * X, p0 * one)
+    o = Oracle(spec)
+    o.set_state(Q)
+    R = o.rhs()
+    _, T, mu, _ = o.primitives()
+    c, sl = _cells(spec), _interior(spec)
+    Ri = R[sl]
+    scale = np.abs(Ri[:, :, 1, :]).max()
+    assert scale > 0
+    # continuity and z-momentum: exactly zero; y-momentum: only the rounding
+    # of the recovered pressure (p = (gamma-1)(E - rho|u|^2/2) varies with u(y)
+    # in its last bits), i.e. c_y * p0 * O(eps)
+    cy = spec.dt * (spec.Lz / n) * (spec.Lx / n)
+    assert np.all(Ri[:, :, 0, :] == 0.0) and np.all(Ri[:, :, 3, :] == 0.0)
+    assert np.abs(Ri[:, :, 2, :]).max() <= 1e-13 * cy * p0
+    dy = spec.Ly / n
+    offs = (np.arange(vo) - vo // 2 + 0.5) * dy
+    s = sum(wm * math.sin(k * o_) for wm, o_ in zip(_D[vo], offs)) / dy  # face derivative of sin -> s * cos
+    y = cases.cell_centres(spec.ny, spec.Ly, spec.order)[spec.order // 2: spec.ny - spec.order // 2]
+    mu0 = mu[c].mean()
+    assert np.ptp(mu[c]) <= 1e-12 * mu0  # uniform T
+    # flux F = -tau = -mu * du/dy at faces j +- 1/2; R = c_y * (F_{j+1/2} - F_{j-1/2})
+    # in QJ units, c_y = dt * dz * dx
+    expect = cy * (-mu0 * U * s) * (np.cos(k * (y + dy / 2)) - np.cos(k * (y - dy / 2)))
+    got = Ri[:, :, 1, :]
+    np.testing.assert_allclose(got, np.broadcast_to(expect[None, :, None], got.shape), rtol=0, atol=1e-8 * scale)
+
+
+# ------------------------------------------------------------ RK tableaux --
+def _rhs_of(spec, QJ):
+    o = Oracle(spec)
+    o.set_qj(QJ)
+    return o.rhs()
+
+
+def test_rk3_shu_osher_from_oracle_rhs():
+    """calc_time_dev.f90.fypp:86-100 with calc_step1/calc_step2_3: Q1 = Q - R(Q),
+    Q2 = 3/4 Q + 1/4 (Q1 - R(Q1)), Q3 = 1/3 Q + 2/3 (Q2 - R(Q2)), composed
+    here from the oracle's right-hand sides (interior cells)."""
+    spec = _spec(10, 4, scheme="SLAU", tvd="hybrid", rk=3)
+    Q = cases.tgv_state(spec)
+    o = Oracle(spec)
+    o.set_state(Q)
+    Q0 = o.get_qj()
+    sl = _interior(spec)
+    fill = lambda A: cases.periodic_fill(A, spec.order)
+    Q1 = Q0.copy()
+    Q1[sl] = Q0[sl] - _rhs_of(spec, Q0)[sl]
+    Q1 = fill(Q1)
+    Q2 = Q1.copy()
+    Q2[sl] = 0.75 * Q0[sl] + 0.25 * Q1[sl] - 0.25 * _rhs_of(spec, Q1)[sl]
+    Q2 = fill(Q2)
+    Q3 = Q2.copy()
+    Q3[sl] = (Q0[sl] + 2.0 * Q2[sl] - 2.0 * _rhs_of(spec, Q2)[sl]) / 3.0
+    o.step(1)
+    ref = o.get_qj()[sl]
+    np.testing.assert_allclose(Q3[sl], ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+    assert np.abs(Q3[sl] - Q0[sl]).max() > 1e-6 * np.abs(ref).max()  # the step moved the state
+
+
+def test_rk4_tableau_from_oracle_rhs():
+    """calc_time_dev.f90.fypp:268-285 (calc_step / calc_step4): classical
+    RK4 with R the full-step residual: Q^{n+1} = Q - (R1 + 2R2 + 2R3 + R4)/6,
+    R1 = R(Q), R2 = R(Q - R1/2), R3 = R(Q - R2/2), R4 = R(Q - R3)."""
+    spec = _spec(10, 4, scheme="KEEP", rk=4)
+    Q = cases.tgv_state(spec)
+    o = Oracle(spec)
+    o.set_state(Q)
+    Q0 = o.get_qj()
+    sl = _interior(spec)
+
+    def stage(base, R, a):
+        X = base.copy()
+        X[sl] = base[sl] - a * R[sl]
+        return cases.periodic_fill(X, spec.order)
+
+    R1 = _rhs_of(spec, Q0)
+    R2 = _rhs_of(spec, stage(Q0, R1, 0.5))
+    R3 = _rhs_of(spec, stage(Q0, R2, 0.5))
+    R4 = _rhs_of(spec, stage(Q0, R3, 1.0))
+    Q4 = Q0[sl] - (R1[sl] + 2 * R2[sl] + 2 * R3[sl] + R4[sl]) / 6.0
+    o.step(1)
+    ref = o.get_qj()[sl]
+    np.testing.assert_allclose(Q4, ref, rtol=0, atol=1e-13 * np.abs(ref).max())

@@ -194,6 +194,8 @@ def initial_state(spec: CaseSpec) -> np.ndarray:
         return tgv_state(spec)
     if spec.ic == "khi_tanh":
         return khi_state(spec)
+    if spec.ic == "dhit":
+        return dhit_state(spec)
     raise ValueError(f"unknown initial condition {spec.ic!r}")
 
 
@@ -258,4 +260,73 @@ def khi_state(spec: CaseSpec) -> np.ndarray:
     ke += v * v
     ke += w * w
     Q[:, :, 4, :] = p / (spec.gamma - 1.0) + 0.5 * rho * ke
+    return periodic_fill(Q, spec.order)
+
+
+# ---------------------------------------------------------------------------
+# Decaying isotropic turbulence (LES): a random-phase solenoidal velocity field
+# built mode by mode in the two-vector frame normal to k (the construction of
+# 3D_solver/DHIT/set_init_dhit.f90:96-169, Rogallo 1981), here with the
+# spectrum E(k) ~ k^4 exp(-2 (k/k0)^2), k0 = 4, on the half spectrum of a real
+# FFT, inverted with numpy's irfftn and scaled to the turbulent Mach number
+# Mt = sqrt(<|u|^2>)/c0 = u' sqrt(3)/c0.  Uniform density and temperature in
+# physical units (the viscosity follows Sutherland's law).
+# ---------------------------------------------------------------------------
+def dhit_spec(n: int, precision: str = "fp32_bulk", seed: int = 20261203, Mt: float = 0.3,
+              cfl: float = 0.3, order: int = 6) -> CaseSpec:
+    N = n + order
+    L = 2.0 * math.pi
+    T0, RHO0 = 300.0, 1.2
+    c0 = math.sqrt(GAMMA * R_GAS * T0)
+    urms = Mt * c0  # sqrt(<|u|^2>)
+    dt = cfl * (L / n) / (2.0 * urms + c0)
+    return CaseSpec(nx=N, ny=N, nz=N, order=order, scheme="Hybrid", visc="LES", visc_order=6 if order == 6 else 4,
+                    tvd="hybrid", slau_variant="HRSLAU2", threshold=0.4, rk=4, precision=precision, dt=dt,
+                    Lx=L, Ly=L, Lz=L, ic="dhit", V0=urms, RHO0=RHO0, T0=T0, p0=RHO0 * R_GAS * T0, seed=seed)
+
+
+def dhit_velocity(n: int, seed: int, k0: float = 4.0):
+    """Solenoidal random-phase velocity (3, n, n, n) in (z, y, x) order with
+    E(k) ~ k^4 exp(-2 (k/k0)^2) (integer wavenumbers of the 2*pi box), not
+    yet scaled."""
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n, d=1.0 / n)
+    kr = np.fft.rfftfreq(n, d=1.0 / n)
+    KZ, KY, KX = np.meshgrid(k1, k1, kr, indexing="ij")
+    kmag = np.sqrt(KX * KX + KY * KY + KZ * KZ)
+    k12 = np.sqrt(KX * KX + KY * KY)
+    nyq = (np.abs(KX) == n // 2) | (np.abs(KY) == n // 2) | (np.abs(KZ) == n // 2)  # no Nyquist modes
+    with np.errstate(divide="ignore", invalid="ignore"):
+        E = kmag ** 4 * np.exp(-2.0 * (kmag / k0) ** 2)
+        amp = np.where((kmag > 0) & ~nyq, np.sqrt(E / (2.0 * np.pi * kmag * kmag)), 0.0)
+        ph = rng.uniform(0.0, 2.0 * np.pi, (3,) + kmag.shape)
+        a = amp * np.exp(1j * ph[0]) * np.cos(ph[2])
+        b = amp * np.exp(1j * ph[1]) * np.sin(ph[2])
+        inplane = k12 > 0
+        s12 = np.where(inplane, k12, 1.0)
+        sk = np.where(kmag > 0, kmag, 1.0)
+        ux = np.where(inplane, (KY / s12) * a + (KX / s12) * (KZ / sk) * b, a)
+        uy = np.where(inplane, -(KX / s12) * a + (KY / s12) * (KZ / sk) * b, 0.0)
+        uz = np.where(inplane, -(k12 / sk) * b, 0.0)
+    try:
+        import scipy.fft as sfft
+        inv = lambda c: sfft.irfftn(c, s=(n, n, n), workers=-1)
+    except ImportError:
+        inv = lambda c: np.fft.irfftn(c, s=(n, n, n))
+    return np.stack([inv(c) for c in (ux, uy, uz)])
+
+
+def dhit_state(spec: CaseSpec) -> np.ndarray:
+    n = spec.nx - spec.order
+    u = dhit_velocity(n, spec.seed)
+    u *= spec.V0 / math.sqrt(float(np.mean(np.sum(u * u, axis=0))))
+    h = spec.order // 2
+    Q = np.zeros(spec.state_shape(), dtype=np.float64)
+    rho, p = spec.RHO0, spec.p0
+    sl = (slice(h, h + n), slice(h, h + n))
+    Q[sl[0], sl[1], 0, h:h + n] = rho
+    Q[sl[0], sl[1], 1, h:h + n] = rho * u[0]
+    Q[sl[0], sl[1], 2, h:h + n] = rho * u[1]
+    Q[sl[0], sl[1], 3, h:h + n] = rho * u[2]
+    Q[sl[0], sl[1], 4, h:h + n] = p / (spec.gamma - 1.0) + 0.5 * rho * np.sum(u * u, axis=0)
     return periodic_fill(Q, spec.order)

@@ -38,7 +38,7 @@ CASES = [
     (6, "SLAU", "LES", 6, "hybrid", "HRSLAU2", 3),
 ]
 FULL = {(4, "KEEP", "NS", 4, "none", "HRSLAU2", 4), (6, "SLAU", "NS", 6, "hybrid", "HRSLAU2", 3)}
-N = (10, 12, 8)
+N = (12, 14, 13)  # each axis >= 2*ORDER, as the library requires
 STEPS = 3
 
 

@@ -35,6 +35,7 @@ CASES = {
                                                         tvd="hybrid", precision=p), 20, 1e-3),
     "roe4-euler-tvd-rk3-ma0.8": (lambda p: cases.tgv_spec(24, 4, Ma=0.8, cfl=0.3, cfl_mode="acoustic", scheme="Roe",
                                                           visc="Euler", rk=3, tvd="tvd", precision=p), 20, 1e-3),
+    "dhit-hybrid6-les6-rk4-mt0.3": (lambda p: cases.dhit_spec(32, p, seed=7), 20, 1e-3),
 }
 
 

@@ -193,6 +193,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-me4", dest="me4", action="store_false",
+                    help="skip the ME4Base-viscous variant of the workload")
     ap.add_argument("--les-n", type=int, default=384,
                     help="interior size of the LES decaying-turbulence leg (0: skip it)")
     ap.add_argument("--les-steps", type=int, default=3)
@@ -265,6 +267,13 @@ def main():
         "kernels": kernels,
         "state_ok": bool(r64["ok"] and r32["ok"]),
     }
+    if args.me4:
+        spec_me4 = cases.config2_spec(args.n, "fp64")
+        spec_me4.visc_stencil = "ME4Base"
+        r_me4 = run_form(spec_me4, args.steps, args.warmup, False)
+        v_me4 = spec_me4.interior_cells * spec_me4.stages * args.steps / r_me4["wall_s"] / 1e6
+        line["viscous_me4base"] = {"value": round(v_me4, 1), "ms_per_step": round(r_me4["wall_s"] * 1e3 / args.steps, 3),
+                                   "form": "FP64 plain, the same workload with the ME4Base viscous stencil"}
     if args.les_n > 0:
         line["les_dhit"] = les_leg(args.les_n, args.les_steps)
     if args.serve_requests > 0:

@@ -25,7 +25,9 @@ bool Oracle::init(const ox_config& cf) {
   if (cf.visc != OX_VISC_EULER && vorder != 4 && vorder != 6) { err = "visc_order must be 4 or 6"; return false; }
   if (cf.visc != OX_VISC_EULER && vorder == 6 && order < 6) { err = "visc_order 6 needs order 6"; return false; }
   if (cf.scheme < OX_SCHEME_KEEP || cf.scheme > OX_SCHEME_HYBRID) { err = "unknown scheme"; return false; }
-  if (cf.visc_stencil != OX_STENCIL_CENTRAL) { err = "ME4Base not in the oracle"; return false; }
+  // ME4Base replaces the central-6 stencil only (calc_visc_high_internal.f90.fypp:17-23);
+  // with VISC_ORDER 4 the central-4 stencil is used whatever VISC_STENCIL says
+  me4 = cf.visc != OX_VISC_EULER && cf.visc_stencil == OX_STENCIL_ME4BASE && vorder == 6;
   if (nx < 2 * order + 2 || ny < 2 * order + 2 || nz < 2 * order + 2) { err = "grid too small"; return false; }
   c.init(cf);
 
@@ -293,7 +295,8 @@ static double cross_deriv(const Oracle& o, int dir, int var, int i, int j, int k
     return o.Qp[o.iq(i, var, j, k + s)];
   };
   const double inv = dir == 0 ? o.xix[i - 1] : (dir == 1 ? o.etay[j - 1] : o.zetaz[k - 1]);
-  if (o.vorder == 4)
+  // 4th-order central: VISC_ORDER 4, and ME4Base (load_smem_visc_me4_base.f90:67-73)
+  if (o.vorder == 4 || o.me4)
     return (c.two_third * (-q(-1) + q(1)) - c.one_twelfth * (-q(-2) + q(2))) * inv;
   return (c.one_sixty * (-q(-3) + q(3)) + 0.15 * (q(-2) - q(2)) + 0.75 * (-q(-1) + q(1))) * inv;
 }
@@ -337,8 +340,45 @@ static void visc_face(const Oracle& o, int dir, int i, int j, int k, double* tau
     }
   }
   const double d = dir == 0 ? o.xix[i - 1] : (dir == 1 ? o.etay[j - 1] : o.zetaz[k - 1]);
-  const double kT = c.Cp_over_Pr * vinterp(N, mu_) * vdiff(c, N, d, T_);
   double ta, tb, tc, ua, ub, uc;
+  if (o.me4) {
+    // calc_visc_high_internal.f90.fypp:146-183 and its F/G counterparts
+    double mu3[3], mut3[3], kT3[3];
+    for (int t = 0; t < 3; ++t) {
+      mu3[t] = me4_interp(mu_, t);
+      if (les) mut3[t] = me4_interp(mt, t);
+      kT3[t] = c.Cp_over_Pr * mu3[t] * (1.125 * (-T_[t + 1] + T_[t + 2]) - (-T_[t] + T_[t + 3]) * c.one_24) * d;
+    }
+    const double kT = flux4(c, kT3);
+    auto straight = [&](const double* vel, const double* a, const double* b, double& t, double& ut) {
+      if (les) tau_straight_me4_les(c, mu3, mut3, vel, a, b, d, t, ut);
+      else tau_straight_me4(c, mu3, vel, a, b, d, t, ut);
+    };
+    auto cross = [&](const double* vel, const double* g, double& t, double& ut) {
+      if (les) tau_cross_me4_les(c, mu3, mut3, vel, g, d, t, ut);
+      else tau_cross_me4(c, mu3, vel, g, d, t, ut);
+    };
+    // SGS heat flux over the 4 central cells (H(i-1:i+2))
+    const double Hsgs =
+        les ? -flux4(c, mut3) * (1.125 * (-H[2] + H[3]) - (-H[1] + H[4]) * c.one_24) * d / c.Prt : 0.0;
+    if (dir == 0) {
+      straight(u_, d2, d4, ta, ua);
+      cross(v_, d1, tb, ub);
+      cross(w_, d3, tc, uc);
+    } else if (dir == 1) {
+      straight(v_, d4, d1, tb, ub);
+      cross(u_, d2, ta, ua);
+      cross(w_, d3, tc, uc);
+    } else {
+      straight(w_, d1, d3, tc, uc);
+      cross(u_, d2, ta, ua);
+      cross(v_, d4, tb, ub);
+    }
+    tau[0] = ta; tau[1] = tb; tau[2] = tc;
+    energy = les ? ua + ub + uc + kT + Hsgs : ua + ub + uc + kT;
+    return;
+  }
+  const double kT = c.Cp_over_Pr * vinterp(N, mu_) * vdiff(c, N, d, T_);
   auto straight = [&](const double* vel, const double* a, const double* b, double& t, double& ut) {
     if (les) tau_straight_les(c, N, mu_, mt, vel, a, b, d, t, ut);
     else tau_straight(c, N, mu_, vel, a, b, d, t, ut);

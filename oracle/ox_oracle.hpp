@@ -451,12 +451,73 @@ inline void tau_cross_les(const Consts& c, int N, const double* mu, const double
 }
 
 // ---------------------------------------------------------------------------
+// ME4Base viscous helpers.  3D_solver/src/calc_visc_me4_base.f90:4-111 and
+// the face assembly of calc_visc_high_internal.f90.fypp:146-183: 4th-order
+// face quantities at the three faces f-1, f, f+1 of a 6-cell window, combined
+// by flux4(a) = (-a1 + 26 a2 - a3)/24.
+// ---------------------------------------------------------------------------
+inline double flux4(const Consts& c, const double* a) { return (-a[0] + 26.0 * a[1] - a[2]) * c.one_24; }
+// 4-point interpolation to face t of a 6-cell window (cells t..t+3)
+inline double me4_interp(const double* a, int t) { return 0.0625 * (9.0 * (a[t + 1] + a[t + 2]) - (a[t] + a[t + 3])); }
+// the same with the operands in the order of the stress and work terms
+inline double me4_interp_w(const double* a, int t) { return 0.0625 * (-a[t] + 9.0 * (a[t + 1] + a[t + 2]) - a[t + 3]); }
+inline void tau_straight_me4(const Consts& c, const double* mu3, const double* u, const double* vy, const double* wz,
+                             double d, double& t11, double& ut11) {
+  double tmp[3], tu[3];
+  for (int t = 0; t < 3; ++t)
+    tmp[t] = c.two_third * mu3[t] *
+             ((2.25 * (-u[t + 1] + u[t + 2]) - (-u[t] + u[t + 3]) * c.one_twelfth) * d - me4_interp_w(vy, t) -
+              me4_interp_w(wz, t));
+  t11 = flux4(c, tmp);
+  for (int t = 0; t < 3; ++t) tu[t] = me4_interp_w(u, t) * tmp[t];
+  ut11 = flux4(c, tu);
+}
+inline void tau_cross_me4(const Consts& c, const double* mu3, const double* v, const double* uy, double d, double& t12,
+                          double& vt12) {
+  double tmp[3], tv[3];
+  for (int t = 0; t < 3; ++t)
+    tmp[t] = mu3[t] * ((1.125 * (-v[t + 1] + v[t + 2]) - (-v[t] + v[t + 3]) * c.one_24) * d + me4_interp_w(uy, t));
+  t12 = flux4(c, tmp);
+  for (int t = 0; t < 3; ++t) tv[t] = me4_interp_w(v, t) * tmp[t];
+  vt12 = flux4(c, tv);
+}
+// LES variants: the work term uses the molecular stress, the stress (mu + mut)
+inline void tau_straight_me4_les(const Consts& c, const double* mu3, const double* mut3, const double* u,
+                                 const double* vy, const double* wz, double d, double& t11, double& ut11) {
+  double t1[3], t2[3];
+  for (int t = 0; t < 3; ++t) {
+    t1[t] = (2.25 * (-u[t + 1] + u[t + 2]) - (-u[t] + u[t + 3]) * c.one_twelfth) * d;
+    t1[t] = t1[t] - me4_interp_w(vy, t);
+    t1[t] = t1[t] - me4_interp_w(wz, t);
+    t2[t] = c.two_third * mu3[t] * t1[t];
+  }
+  for (int t = 0; t < 3; ++t) t2[t] = me4_interp_w(u, t) * t2[t];
+  ut11 = flux4(c, t2);
+  for (int t = 0; t < 3; ++t) t2[t] = c.two_third * (mu3[t] + mut3[t]) * t1[t];
+  t11 = flux4(c, t2);
+}
+inline void tau_cross_me4_les(const Consts& c, const double* mu3, const double* mut3, const double* v,
+                              const double* uy, double d, double& t12, double& vt12) {
+  double t1[3], t2[3];
+  for (int t = 0; t < 3; ++t) {
+    t1[t] = (1.125 * (-v[t + 1] + v[t + 2]) - (-v[t] + v[t + 3]) * c.one_24) * d;
+    t1[t] = t1[t] + me4_interp_w(uy, t);
+    t2[t] = mu3[t] * t1[t];
+  }
+  for (int t = 0; t < 3; ++t) t2[t] = me4_interp_w(v, t) * t2[t];
+  vt12 = flux4(c, t2);
+  for (int t = 0; t < 3; ++t) t2[t] = (mu3[t] + mut3[t]) * t1[t];
+  t12 = flux4(c, t2);
+}
+
+// ---------------------------------------------------------------------------
 // The solver state.
 // ---------------------------------------------------------------------------
 struct Oracle {
   ox_config cfg{};
   Consts c{};
   int nx = 0, ny = 0, nz = 0, order = 0, io = 0, off = 0, vorder = 0;
+  bool me4 = false;  // ME4Base viscous stencil (VISC_ORDER 6, VISC_STENCIL 'ME4Base')
   std::vector<double> dx, dy, dz;        // grid spacings (n-1)
   std::vector<double> xix, etay, zetaz;  // 1/dx etc. (n-1)
   std::vector<double> Jac;               // (nx,ny)

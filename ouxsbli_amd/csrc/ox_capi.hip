@@ -138,8 +138,8 @@ int validate(const ox_config& c) {
   if (c.order != 2 && c.order != 4 && c.order != 6) return ox_fail(OX_EINVAL, "order must be 2, 4 or 6");
   if (c.rk != 3 && c.rk != 4) return ox_fail(OX_EINVAL, "rk must be 3 or 4");
   if (c.bc_x || c.bc_y || c.bc_z) return ox_fail(OX_EUNSUPPORTED, "only periodic boundaries are built");
-  if (c.visc_stencil != OX_STENCIL_CENTRAL && c.visc != OX_VISC_EULER)
-    return ox_fail(OX_EUNSUPPORTED, "ME4Base viscous stencil is not built yet");
+  if (c.visc_stencil != OX_STENCIL_CENTRAL && c.visc_stencil != OX_STENCIL_ME4BASE)
+    return ox_fail(OX_EINVAL, "unknown viscous stencil");
   if (c.scheme < OX_SCHEME_KEEP || c.scheme > OX_SCHEME_HYBRID) return ox_fail(OX_EINVAL, "unknown scheme");
   if (c.tvd < 0 || c.tvd > 2) return ox_fail(OX_EINVAL, "tvd must be 0, 1 or 2");
   if (c.slau_variant < 0 || c.slau_variant > 1) return ox_fail(OX_EINVAL, "slau_variant must be 0 or 1");
@@ -232,6 +232,9 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   C.hc.init(*cfg);
   C.hc.les_delta15 = ox::sms_delta15(C.g.inv_dx, C.g.inv_dy, C.g.inv_dz);
   C.vorder = cfg->visc == OX_VISC_EULER ? 0 : (cfg->visc_order ? cfg->visc_order : cfg->order);
+  // ME4Base replaces the central-6 stencil only (calc_visc_high_internal.f90.fypp:17-23):
+  // internal stencil code 5
+  const bool me4 = C.vorder == 6 && cfg->visc_stencil == OX_STENCIL_ME4BASE;
   C.tvd = cfg->tvd;
   C.variant = cfg->slau_variant;
   C.Racc = nullptr;
@@ -252,6 +255,7 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
     delete s;
     return ox_fail(OX_EINVAL, "visc_order 6 needs order 6 (three ghost layers)");
   }
+  if (me4) C.vorder = 5;
   s->stage = cfg->precision == OX_PREC_FP64 ? ox::find_stage_fp64(cfg->order, cfg->scheme, cfg->visc, C.vorder)
                                             : ox::find_stage_fp32(cfg->order, cfg->scheme, cfg->visc, C.vorder);
   if (!s->stage) {

@@ -38,14 +38,20 @@ template <int SCHEME, int ORDER> constexpr bool kUsesSensor = SCHEME == HYBRID |
 // schemes whose face flux reads T (KEEP's internal-energy term)
 template <int SCHEME> constexpr bool kUsesT = SCHEME == KEEP || SCHEME == HYBRID;
 
+// Viscous stencil codes VO: 4 and 6 are the central stencils of that order
+// (calc_visc_cent.f90.fypp); 5 is ME4Base (calc_visc_me4_base.f90): a 6-cell
+// face window with 4th-order cross derivatives.
+template <int VO> constexpr int vwin() { return VO == 5 ? 6 : VO; }    // face window (cells)
+template <int VO> constexpr int vcross() { return VO == 5 ? 4 : VO; }  // cross-derivative order
+
 // Stencil width of a face: the convective stencil (ORDER cells) and the
-// viscous one (VO cells) are both centred on the face.
+// viscous one (vwin cells) are both centred on the face.
 template <int ORDER, int VISC, int VO> struct Sten {
-  static constexpr int SW = (VISC && VO > ORDER) ? VO : ORDER;
+  static constexpr int SW = (VISC && vwin<VO>() > ORDER) ? vwin<VO>() : ORDER;
   static constexpr int HW = SW / 2;
   static constexpr int CO = HW - ORDER / 2;  // first convective cell in the window
-  static constexpr int VOFF = VISC ? HW - VO / 2 : 0;
-  static constexpr int VH = VO / 2;  // half width of the cross-derivative stencil
+  static constexpr int VOFF = VISC ? HW - vwin<VO>() / 2 : 0;
+  static constexpr int VH = vcross<VO>() / 2;  // half width of the cross-derivative stencil
 };
 
 // state index of (i,l,j,k), 0-based
@@ -238,7 +244,7 @@ template <class R, int DIR, int ORDER, int SCHEME, int VISC, int VO> struct Cell
 // separate from CellData so that a sweep can issue the loads of the next cell
 // one face ahead of their use.
 template <class S, int VISC, int VO> struct RawCell {
-  static constexpr int NX = VISC ? VO : 1;
+  static constexpr int NX = VISC ? vcross<VO>() : 1;
   S rho, u, v, w, p, T, mu, s, mt, qc2;
   S x[4][NX];
 };
@@ -276,11 +282,12 @@ __device__ __forceinline__ void fetch_cell(const Prim<S>& P, const Geom& g, int 
     const S* f[4];
     int st[4];
     cross_fields<S, DIR>(P, g, f, st);
+    constexpr int VC = vcross<VO>();
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
 #pragma unroll
-      for (int t = 0; t < VO; ++t) {
-        const int off = t < VO / 2 ? t - VO / 2 : t - VO / 2 + 1;
+      for (int t = 0; t < VC; ++t) {
+        const int off = t < VC / 2 ? t - VC / 2 : t - VC / 2 + 1;
         r.x[q][t] = f[q][m + off * st[q]];
       }
     }
@@ -290,10 +297,11 @@ __device__ __forceinline__ void fetch_cell(const Prim<S>& P, const Geom& g, int 
 // cross_d over the raw stencil values (centre entry zero, never read)
 template <class R, class S, int VO>
 __device__ __forceinline__ R cross_raw(const PhysC<R>& c, const S* x, R inv) {
-  R a[VO + 1];
+  constexpr int VC = vcross<VO>();
+  R a[VC + 1];
 #pragma unroll
-  for (int t = 0; t <= VO; ++t) a[t] = (t == VO / 2) ? R(0) : R(x[t < VO / 2 ? t : t - 1]);
-  return cross_d<R, VO>(c, inv, a);
+  for (int t = 0; t <= VC; ++t) a[t] = (t == VC / 2) ? R(0) : R(x[t < VC / 2 ? t : t - 1]);
+  return cross_d<R, VC>(c, inv, a);
 }
 
 // Turns the loaded values into the window cell (conversions, cross
@@ -370,7 +378,7 @@ __device__ __forceinline__ void face_flux(const PhysC<R>& c, int tvd, int varian
     }
   }
   if constexpr (VISC) {
-    constexpr int M = VO;
+    constexpr int M = vwin<VO>();
     R mu_[M], T_[M], u_[M], v_[M], w_[M], a1[M], a2[M], a3[M], a4[M], mt_[M], H_[M];
 #pragma unroll
     for (int m = 0; m < M; ++m) {
@@ -380,7 +388,8 @@ __device__ __forceinline__ void face_flux(const PhysC<R>& c, int tvd, int varian
       if constexpr (VISC == 2) { mt_[m] = x.mt; H_[m] = x.H; }
     }
     R tau[3], en;
-    visc_face<R, DIR, M, (VISC == 2)>(c, inv_d, mu_, T_, u_, v_, w_, a1, a2, a3, a4, tau, en, mt_, H_);
+    if constexpr (VO == 5) visc_face_me4<R, DIR, (VISC == 2)>(c, inv_d, mu_, T_, u_, v_, w_, a1, a2, a3, a4, tau, en, mt_, H_);
+    else visc_face<R, DIR, M, (VISC == 2)>(c, inv_d, mu_, T_, u_, v_, w_, a1, a2, a3, a4, tau, en, mt_, H_);
     F[1] = F[1] - tau[0];
     F[2] = F[2] - tau[1];
     F[3] = F[3] - tau[2];
@@ -539,7 +548,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<R, S>& a, int bx, i
     fetch_cell<S, 0, ORDER, SCHEME, VISC, VO, false>(P, g, q, 0, r);
     if constexpr (SPLIT_SENS) {
       if (spos) {
-        constexpr int H = VO / 2;  // r.x[.][H-1], r.x[.][H]: the -1 and +1 neighbours
+        constexpr int H = vcross<VO>() / 2;  // r.x[.][H-1], r.x[.][H]: the -1 and +1 neighbours
         R uym, uyp, vym, vyp, uzm, uzp, wzm, wzp;
         if constexpr (VISC) {  // DIR 0 cross stencils: u,v along y; u,w along z
           uym = R(r.x[0][H - 1]); uyp = R(r.x[0][H]);

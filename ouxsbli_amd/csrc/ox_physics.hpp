@@ -478,6 +478,121 @@ OX_HD void visc_face(const PhysC<R>& c, R d, const R* mu, const R* T, const R* u
   if constexpr (LES) energy = energy + Hsgs;
 }
 
+// ---------------------------------------------------------------------------
+// ME4Base viscous face.  3D_solver/src/calc_visc_me4_base.f90:4-111 with the
+// face assembly of calc_visc_high_internal.f90.fypp:146-183 (and the F/G
+// counterparts): mu (and mut) interpolated to the three faces f-1, f, f+1 of
+// the 6-cell window, 4th-order face derivatives there, and the three face
+// values combined by flux4(a) = (-a1 + 26 a2 - a3)/24.  The LES stress uses
+// (mu + mut) here, the work terms the molecular stress.
+// ---------------------------------------------------------------------------
+template <class R> OX_HD R flux4(const PhysC<R>& c, R a0, R a1, R a2) { return (-a0 + R(26) * a1 - a2) * c.one_24; }
+template <class R> OX_HD R me4_interp(const R* a, int t) {
+  return R(0.0625) * (R(9) * (a[t + 1] + a[t + 2]) - (a[t] + a[t + 3]));
+}
+template <class R> OX_HD R me4_interp_w(const R* a, int t) {
+  return R(0.0625) * (-a[t] + R(9) * (a[t + 1] + a[t + 2]) - a[t + 3]);
+}
+template <class R, bool LES>
+OX_HD void me4_straight(const PhysC<R>& c, const R* mu3, const R* mut3, const R* u, const R* sa, const R* sb, R d,
+                        R& ts, R& uts) {
+  R t1[3], t2[3];
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    if constexpr (!LES) {
+      t1[t] = c.two_third * mu3[t] *
+              ((R(2.25) * (-u[t + 1] + u[t + 2]) - (-u[t] + u[t + 3]) * c.one_twelfth) * d - me4_interp_w(sa, t) -
+               me4_interp_w(sb, t));
+    } else {
+      t1[t] = (R(2.25) * (-u[t + 1] + u[t + 2]) - (-u[t] + u[t + 3]) * c.one_twelfth) * d;
+      t1[t] = t1[t] - me4_interp_w(sa, t);
+      t1[t] = t1[t] - me4_interp_w(sb, t);
+      t2[t] = c.two_third * mu3[t] * t1[t];
+    }
+  }
+  if constexpr (!LES) {
+    ts = flux4(c, t1[0], t1[1], t1[2]);
+#pragma unroll
+    for (int t = 0; t < 3; ++t) t2[t] = me4_interp_w(u, t) * t1[t];
+    uts = flux4(c, t2[0], t2[1], t2[2]);
+  } else {
+#pragma unroll
+    for (int t = 0; t < 3; ++t) t2[t] = me4_interp_w(u, t) * t2[t];
+    uts = flux4(c, t2[0], t2[1], t2[2]);
+#pragma unroll
+    for (int t = 0; t < 3; ++t) t2[t] = c.two_third * (mu3[t] + mut3[t]) * t1[t];
+    ts = flux4(c, t2[0], t2[1], t2[2]);
+  }
+}
+template <class R, bool LES>
+OX_HD void me4_cross(const PhysC<R>& c, const R* mu3, const R* mut3, const R* v, const R* g, R d, R& t12, R& vt12) {
+  R t1[3], t2[3];
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    if constexpr (!LES) {
+      t1[t] = mu3[t] * ((R(1.125) * (-v[t + 1] + v[t + 2]) - (-v[t] + v[t + 3]) * c.one_24) * d + me4_interp_w(g, t));
+    } else {
+      t1[t] = (R(1.125) * (-v[t + 1] + v[t + 2]) - (-v[t] + v[t + 3]) * c.one_24) * d;
+      t1[t] = t1[t] + me4_interp_w(g, t);
+      t2[t] = mu3[t] * t1[t];
+    }
+  }
+  if constexpr (!LES) {
+    t12 = flux4(c, t1[0], t1[1], t1[2]);
+#pragma unroll
+    for (int t = 0; t < 3; ++t) t2[t] = me4_interp_w(v, t) * t1[t];
+    vt12 = flux4(c, t2[0], t2[1], t2[2]);
+  } else {
+#pragma unroll
+    for (int t = 0; t < 3; ++t) t2[t] = me4_interp_w(v, t) * t2[t];
+    vt12 = flux4(c, t2[0], t2[1], t2[2]);
+#pragma unroll
+    for (int t = 0; t < 3; ++t) t2[t] = (mu3[t] + mut3[t]) * t1[t];
+    t12 = flux4(c, t2[0], t2[1], t2[2]);
+  }
+}
+
+template <class R, int DIR, bool LES = false>
+OX_HD void visc_face_me4(const PhysC<R>& c, R d, const R* mu, const R* T, const R* u, const R* v, const R* w,
+                         const R* d1, const R* d2, const R* d3, const R* d4, R* tau, R& energy,
+                         const R* mt = nullptr, const R* H = nullptr) {
+  R mu3[3], mut3[3], kT3[3];
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    mu3[t] = me4_interp(mu, t);
+    if constexpr (LES) mut3[t] = me4_interp(mt, t);
+    kT3[t] = c.Cp_over_Pr * mu3[t] * (R(1.125) * (-T[t + 1] + T[t + 2]) - (-T[t] + T[t + 3]) * c.one_24) * d;
+  }
+  const R kT = flux4(c, kT3[0], kT3[1], kT3[2]);
+  const R* vs = DIR == 0 ? u : (DIR == 1 ? v : w);
+  const R* sa = DIR == 0 ? d2 : (DIR == 1 ? d4 : d1);  // vy | wz | ux
+  const R* sb = DIR == 0 ? d4 : (DIR == 1 ? d1 : d3);  // wz | ux | vy
+  const R* va = DIR == 0 ? v : u;
+  const R* ga = DIR == 0 ? d1 : d2;                    // uy | vx | wx
+  const R* vb = DIR == 2 ? v : w;
+  const R* gb = DIR == 0 ? d3 : (DIR == 1 ? d3 : d4);  // uz | vz | wy
+  R ts, uts, ta, uta, tb, utb;
+  me4_straight<R, LES>(c, mu3, mut3, vs, sa, sb, d, ts, uts);
+  me4_cross<R, LES>(c, mu3, mut3, va, ga, d, ta, uta);
+  me4_cross<R, LES>(c, mu3, mut3, vb, gb, d, tb, utb);
+  if constexpr (DIR == 0) {
+    tau[0] = ts; tau[1] = ta; tau[2] = tb;
+    energy = uts + uta + utb + kT;
+  } else if constexpr (DIR == 1) {
+    tau[0] = ta; tau[1] = ts; tau[2] = tb;
+    energy = uta + uts + utb + kT;
+  } else {
+    tau[0] = ta; tau[1] = tb; tau[2] = ts;
+    energy = uta + utb + uts + kT;
+  }
+  if constexpr (LES) {
+    // SGS heat flux over the four central cells of the window
+    const R Hsgs = -flux4(c, mut3[0], mut3[1], mut3[2]) * (R(1.125) * (-H[2] + H[3]) - (-H[1] + H[4]) * c.one_24) *
+                   d / c.Prt;
+    energy = energy + Hsgs;
+  }
+}
+
 // Cross derivative stencils of the LDS loaders
 // (3D_solver/src/load_smem_visc_cent.f90.fypp:52-66): f[m] = value at offset
 // m - VO/2 ... (centre excluded is never read).

@@ -163,14 +163,19 @@ template <class R, class S, int ORDER, int SCHEME, int VISC, int VO> BatchOps ba
   X(6, HYBRID, 0, 0)         \
   X(4, HYBRID, 1, 4)         \
   X(6, HYBRID, 1, 4)         \
-  X(6, HYBRID, 1, 6)
-// ... and with the LES model (single-case launches only)
+  X(6, HYBRID, 1, 6)         \
+  X(6, KEEP, 1, 5)           \
+  X(6, SLAU, 1, 5)           \
+  X(6, HYBRID, 1, 5)
+// ... and with the LES model (single-case launches only).  VO 5 is ME4Base.
 #define OX_FOR_EACH_LES_STAGE(X) \
   X(4, KEEP, 2, 4)               \
   X(6, KEEP, 2, 6)               \
   X(6, SLAU, 2, 6)               \
   X(4, HYBRID, 2, 4)             \
-  X(6, HYBRID, 2, 6)
+  X(6, HYBRID, 2, 6)             \
+  X(6, SLAU, 2, 5)               \
+  X(6, HYBRID, 2, 5)
 
 // Table bodies: R_ and S_ name the arithmetic and storage types.
 #define OX_STAGE_CASE(O, SC, V, VO) \

@@ -23,7 +23,7 @@ sys.path.insert(0, str(HERE.parent.parent))
 
 from ouxsbli_amd import cases  # noqa: E402
 
-# (order, scheme, visc, visc_order, tvd, slau_variant, rk, precision)
+# (order, scheme, visc, visc_order, tvd, slau_variant, rk[, viscous stencil])
 CASES = [
     (2, "KEEP", "Euler", 0, "none", "HRSLAU2", 4),
     (4, "KEEP", "NS", 4, "none", "HRSLAU2", 4),
@@ -36,6 +36,8 @@ CASES = [
     (4, "Hybrid", "NS", 4, "tvd", "HRSLAU2", 4),
     (6, "Hybrid", "LES", 6, "hybrid", "HRSLAU2", 4),
     (6, "SLAU", "LES", 6, "hybrid", "HRSLAU2", 3),
+    (6, "SLAU", "NS", 6, "hybrid", "HRSLAU2", 3, "ME4Base"),
+    (6, "Hybrid", "LES", 6, "hybrid", "HRSLAU2", 4, "ME4Base"),
 ]
 FULL = {(4, "KEEP", "NS", 4, "none", "HRSLAU2", 4), (6, "SLAU", "NS", 6, "hybrid", "HRSLAU2", 3)}
 N = (12, 14, 13)  # each axis >= 2*ORDER, as the library requires
@@ -47,9 +49,10 @@ def case_id(c):
 
 
 def spec_of(c):
-    order, scheme, visc, vo, tvd, slau, rk = c
+    order, scheme, visc, vo, tvd, slau, rk = c[:7]
+    stencil = c[7] if len(c) > 7 else "Central"
     s = cases.tgv_spec(N[0], order, Ma=0.8, cfl=0.3, cfl_mode="acoustic", scheme=scheme, visc=visc,
-                       visc_order=vo, tvd=tvd, slau_variant=slau, rk=rk)
+                       visc_order=vo, visc_stencil=stencil, tvd=tvd, slau_variant=slau, rk=rk)
     s.ny, s.nz = N[1] + order, N[2] + order
     s.Ly, s.Lz = s.Lx * N[1] / N[0], s.Lx * N[2] / N[0]
     return s

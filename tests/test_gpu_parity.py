@@ -53,12 +53,18 @@ CASES = [
     (6, "Hybrid", "LES", 6, "hybrid", "HRSLAU2", 4),
     (6, "SLAU", "LES", 6, "hybrid", "HRSLAU2", 3),
     (4, "KEEP", "LES", 4, "none", "HRSLAU2", 4),
+    # ME4Base viscous stencil (VISC_ORDER 6, VISC_STENCIL 'ME4Base')
+    (6, "SLAU", "NS", 6, "hybrid", "HRSLAU2", 3, "ME4Base"),
+    (6, "KEEP", "NS", 6, "none", "HRSLAU2", 4, "ME4Base"),
+    (6, "Hybrid", "NS", 6, "tvd", "SLAU", 3, "ME4Base"),
+    (6, "Hybrid", "LES", 6, "hybrid", "HRSLAU2", 4, "ME4Base"),
+    (6, "SLAU", "LES", 6, "hybrid", "HRSLAU2", 3, "ME4Base"),
 ]
 
 
-def _spec(order, scheme, visc, vo, tvd, slau, rk, n=(20, 18, 22), Ma=0.8):
+def _spec(order, scheme, visc, vo, tvd, slau, rk, stencil="Central", n=(20, 18, 22), Ma=0.8):
     s = cases.tgv_spec(n[0], order, Ma=Ma, cfl=0.3, cfl_mode="acoustic", scheme=scheme, visc=visc,
-                       visc_order=vo, tvd=tvd, slau_variant=slau, rk=rk)
+                       visc_order=vo, visc_stencil=stencil, tvd=tvd, slau_variant=slau, rk=rk)
     s.ny = n[1] + order
     s.nz = n[2] + order
     s.Ly = s.Lx * n[1] / n[0]

@@ -195,10 +195,12 @@ def _small(order=4, scheme="KEEP", visc="NS", n=12, **kw):
     return cases.tgv_spec(n, order, Ma=0.3, cfl=0.3, cfl_mode="acoustic", scheme=scheme, visc=visc, **kw)
 
 
-@pytest.mark.parametrize("scheme,order,visc", [("KEEP", 4, "NS"), ("KEEP", 6, "NS"), ("SLAU", 6, "NS"),
-                                               ("SLAU", 4, "Euler"), ("KEEP", 2, "Euler")])
-def test_free_stream_preserved(scheme, order, visc):
-    spec = _small(order, scheme, visc)
+@pytest.mark.parametrize("scheme,order,visc,stencil", [("KEEP", 4, "NS", "Central"), ("KEEP", 6, "NS", "Central"),
+                                                       ("SLAU", 6, "NS", "Central"), ("SLAU", 4, "Euler", "Central"),
+                                                       ("KEEP", 2, "Euler", "Central"), ("SLAU", 6, "NS", "ME4Base"),
+                                                       ("Hybrid", 6, "LES", "ME4Base")])
+def test_free_stream_preserved(scheme, order, visc, stencil):
+    spec = _small(order, scheme, visc, visc_stencil=stencil)
     Q = np.zeros(spec.state_shape())
     Q[:, :, 0, :] = 1.2
     Q[:, :, 1, :] = 1.2 * 30.0
@@ -210,11 +212,13 @@ def test_free_stream_preserved(scheme, order, visc):
     assert np.abs(o.rhs()).max() == 0.0
 
 
-@pytest.mark.parametrize("scheme,order,visc,tvd", [("KEEP", 4, "NS", "none"), ("SLAU", 6, "NS", "hybrid"),
-                                                   ("SLAU", 4, "Euler", "tvd")])
-def test_periodic_conservation(scheme, order, visc, tvd):
+@pytest.mark.parametrize("scheme,order,visc,tvd,stencil", [("KEEP", 4, "NS", "none", "Central"),
+                                                           ("SLAU", 6, "NS", "hybrid", "Central"),
+                                                           ("SLAU", 4, "Euler", "tvd", "Central"),
+                                                           ("KEEP", 6, "NS", "none", "ME4Base")])
+def test_periodic_conservation(scheme, order, visc, tvd, stencil):
     """Sum over the interior of the RHS telescopes to zero (mass, momentum, energy)."""
-    spec = _small(order, scheme, visc, tvd=tvd, Ma=0.9) if False else _small(order, scheme, visc, tvd=tvd)
+    spec = _small(order, scheme, visc, tvd=tvd, visc_stencil=stencil)
     o = Oracle(spec)
     o.set_state(cases.tgv_state(spec))
     R = o.rhs()
@@ -314,9 +318,10 @@ def test_time_order(rk, p):
     assert p - 0.5 <= rate <= p + 0.8, (e1, e2, rate)
 
 
-def _shear_error(n, vo, order):
+def _shear_error(n, vo, order, stencil="Central"):
     s = cases.CaseSpec(nx=2 * order + 2, ny=n + order, nz=2 * order + 2, order=order, scheme="KEEP", visc="NS",
-                       visc_order=vo, rk=4, Lx=(order + 2) / n, Ly=1.0, Lz=(order + 2) / n, dt=1e-3)
+                       visc_order=vo, visc_stencil=stencil, rk=4, Lx=(order + 2) / n, Ly=1.0, Lz=(order + 2) / n,
+                       dt=1e-3)
     y = cases.cell_centres(s.ny, s.Ly, order)
     rho, p, U = 1.0, 1.0e5, 2.0
     k = 2 * np.pi
@@ -333,22 +338,28 @@ def _shear_error(n, vo, order):
     mu = 1.716e-5 * 384.2 * 273.2 ** -1.5 / (T + 111.0) * T ** 1.5  # Sutherland, mod_constant.f90.fypp:26
     dx, dy, dz = s.Lx / (s.nx - order), s.Ly / (s.ny - order), s.Lz / (s.nz - order)
     h = order // 2
-    # R = dt * A * div(F);  d(rho u)/dt * vol = -R/dt.  The scheme differences
-    # face stresses, so the semi-discrete target is mu*(u'(y+dy/2) - u'(y-dy/2))/dy.
+    # R = dt * A * div(F);  d(rho u)/dt * vol = -R/dt.  The central scheme
+    # differences face stresses, so its semi-discrete target is
+    # mu*(u'(y+dy/2) - u'(y-dy/2))/dy.  ME4Base deconvolves its face fluxes
+    # (flux4 = F - dy^2/24 F''), so its target is the pointwise mu*u''.
     rate = -R[h:-h, h:-h, 1, h:-h] / (s.dt * dx * dy * dz)
     yi = y[h:-h]
-    expected = mu * U * k * (np.cos(k * (yi + dy / 2)) - np.cos(k * (yi - dy / 2))) / dy
+    if stencil == "ME4Base":
+        expected = -mu * U * k * k * np.sin(k * yi)
+    else:
+        expected = mu * U * k * (np.cos(k * (yi + dy / 2)) - np.cos(k * (yi - dy / 2))) / dy
     return np.abs(rate[0, :, 0] - expected).max() / np.abs(expected).max()
 
 
-@pytest.mark.parametrize("vo,order,min_rate", [(4, 4, 3.6), (6, 6, 5.6), (4, 6, 3.6)])
-def test_viscous_shear_rate(vo, order, min_rate):
+@pytest.mark.parametrize("vo,order,min_rate,stencil", [(4, 4, 3.6, "Central"), (6, 6, 5.6, "Central"),
+                                                       (4, 6, 3.6, "Central"), (6, 6, 3.6, "ME4Base")])
+def test_viscous_shear_rate(vo, order, min_rate, stencil):
     """u = U sin(k y) in uniform rho, p: the x-momentum RHS is the viscous term
     (central viscous operator, calc_visc_cent.f90.fypp).  Its face stress is a
     VO-th order staggered derivative, so the difference of exact face
     stresses is matched at that order; mu is Sutherland's law."""
-    e1 = _shear_error(24, vo, order)
-    e2 = _shear_error(48, vo, order)
+    e1 = _shear_error(24, vo, order, stencil)
+    e2 = _shear_error(48, vo, order, stencil)
     assert e1 < 1e-2
     assert math.log(e1 / e2, 2) >= min_rate, (e1, e2)
 

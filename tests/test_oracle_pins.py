@@ -118,7 +118,7 @@ _D = {4: [1 / 24, -27 / 24, 27 / 24, -1 / 24],
       6: [-3 / 640, 25 / 384, -75 / 64, 75 / 64, -25 / 384, 3 / 640]}
 
 
-@pytest.mark.parametrize("vo", [4, 6])
+@pytest.mark.parametrize("vo", [4, 6, "ME4Base"])
 def test_viscous_shear_wave_discrete_symbol(vo):
     """u = U sin(k y), v = w = 0, uniform rho and p: every convective flux
     difference vanishes, the continuity and y/z-momentum residuals are zero,
@@ -129,10 +129,12 @@ def test_viscous_shear_wave_discrete_symbol(vo):
     s(k) = sum_m w_m sin(k (m - VO/2 + 1/2) dy) / dy.  The reference's
     truncated literal 0.065104167 (for 25/384) is within the 1e-8 tolerance;
     a wrong weight, sign or index is not."""
-    order = 6 if vo == 6 else 4
+    me4 = vo == "ME4Base"
+    order = 4 if vo == 4 else 6
     n = 24
     spec = cases.tgv_spec(n, order, Ma=0.3, cfl=0.3, cfl_mode="acoustic", scheme="KEEP", visc="NS",
-                          visc_order=vo, tvd="none", rk=3)
+                          visc_order=6 if me4 else vo, visc_stencil="ME4Base" if me4 else "Central", tvd="none",
+                          rk=3)
     Z, Y, X = _grid(spec)
     one = np.ones_like(X)
     k = 3.0 * 2 * math.pi / spec.Ly
@@ -154,8 +156,13 @@ def test_viscous_shear_wave_discrete_symbol(vo):
     assert np.all(Ri[:, :, 0, :] == 0.0) and np.all(Ri[:, :, 3, :] == 0.0)
     assert np.abs(Ri[:, :, 2, :]).max() <= 1e-13 * cy * p0
     dy = spec.Ly / n
-    offs = (np.arange(vo) - vo // 2 + 0.5) * dy
-    s = sum(wm * math.sin(k * o_) for wm, o_ in zip(_D[vo], offs)) / dy  # face derivative of sin -> s * cos
+    w = 4 if me4 else vo
+    offs = (np.arange(w) - w // 2 + 0.5) * dy
+    s = sum(wm * math.sin(k * o_) for wm, o_ in zip(_D[w], offs)) / dy  # face derivative of sin -> s * cos
+    if me4:
+        # ME4Base (calc_visc_me4_base.f90): 4th-order face stresses at faces
+        # f-1, f, f+1 combined by (-a + 26 b - c)/24, i.e. times (26 - 2 cos k dy)/24
+        s *= (26.0 - 2.0 * math.cos(k * dy)) / 24.0
     y = cases.cell_centres(spec.ny, spec.Ly, spec.order)[spec.order // 2: spec.ny - spec.order // 2]
     mu0 = mu[c].mean()
     assert np.ptp(mu[c]) <= 1e-12 * mu0  # uniform T

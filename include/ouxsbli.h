@@ -256,6 +256,78 @@ int ox_batch_retire(ox_batch* b, int64_t id, double* Q);
 
 int ox_batch_query(ox_batch* b, ox_batch_stats* out);
 
+/* ---------------------------------------------------------------------------
+ * Two-dimensional cases with boundaries.  The reference builds these from
+ * 2D_solver/src (one executable per case directory); the case implemented
+ * here is its inviscid oblique-shock reflection, 2D_solver/OS.
+ *
+ *   - State Q(nx,4,ny) in Fortran order (x fastest, then the variable, then
+ *     y): element (i,l,j) (1-based) at (i-1) + nx*((l-1) + 4*(j-1)).
+ *     Variables are (rho, rho*u, rho*v, E).
+ *   - nx, ny count one boundary layer on each side.  Interior points are
+ *     2..nx-1 and 2..ny-1, on the uniform grid x = (i-1)*Lx/(nx-1),
+ *     y = (j-1)*Ly/(ny-1) (2D_solver/OS/set.f90:8-26).
+ *   - Faces next to a boundary fall back to lower order: ORDER 6 uses the
+ *     6-point MUSCL for faces 3..n-3, the 4-point one for faces 2 and n-2, and
+ *     first order for faces 1 and n-1 (2D_solver/src/calc_slau_kernel.f90.fypp:17-200).
+ *   - After every RK stage the boundary points take the oblique-shock
+ *     conditions of 2D_solver/OS/set.f90:48-86:
+ *       left column      the inflow state;
+ *       top row          inflow for i < top_split, the post-shock state otherwise;
+ *       bottom row       slip wall (copy of row 2 with v negated);
+ *       right column     copy of column nx-1.
+ * ------------------------------------------------------------------------- */
+#define OX2D_BC_OBLIQUE_SHOCK 1
+
+typedef struct ox2d_config {
+  int32_t abi_version;   /* OX_ABI_VERSION                                          */
+  int32_t nx, ny;        /* points incl. the boundary layer on each side            */
+  int32_t order;         /* ORDER: 2, 4 or 6                                        */
+  int32_t scheme;        /* OX_SCHEME_SLAU or OX_SCHEME_ROE                         */
+  int32_t tvd;           /* OX_TVD_*                                                */
+  int32_t slau_variant;  /* OX_SLAU_*                                               */
+  int32_t rk;            /* 3 (TVD, Shu-Osher)                                      */
+  int32_t precision;     /* OX_PREC_*                                               */
+  int32_t bc;            /* OX2D_BC_OBLIQUE_SHOCK                                   */
+  int32_t top_split;     /* 1-based; int(0.1*nx) in OS/set.f90:55                   */
+  int32_t device;        /* HIP device ordinal                                      */
+  double gamma, R;
+  double threshold;      /* hybrid MUSCL switch on the Ducros sensor                */
+  double dt;
+  double Lx, Ly;
+  double inflow[4];      /* rho, u, v, p of the left boundary and the top for i < top_split */
+  double post[4];        /* rho, u, v, p of the top for i >= top_split              */
+} ox2d_config;
+
+typedef struct ox2d_solver ox2d_solver;
+
+/* Creates a 2-D case: device state, grid metrics.  Replaces
+ * allocate_device_mem + pre_calc (2D_solver/src/preprocess.f90.fypp:25-111),
+ * set_grid (2D_solver/OS/set.f90:8-26) and set_Jacobian_xy2
+ * (src/set_coordinate.f90:78-96). */
+int ox2d_create(const ox2d_config* cfg, ox2d_solver** out);
+void ox2d_destroy(ox2d_solver* s);
+
+/* Physical conserved state Q(nx,4,ny) in and out (QJ = Q / J on the way in,
+ * as pre_calc, preprocess.f90.fypp:86-92; J * QJ on the way out, as
+ * 2D_solver/src/main.f90:80-85). */
+int ox2d_set_state(ox2d_solver* s, const double* Q);
+int ox2d_get_state(ox2d_solver* s, double* Q);
+
+/* Advances nsteps RK3 steps, each stage followed by the boundary conditions.
+ * Replaces the `t1` loop of RungeKutta (2D_solver/src/calc_time_dev.f90.fypp:77-94). */
+int ox2d_step(ox2d_solver* s, int32_t nsteps);
+
+/* Device time of the last ox2d_step call in milliseconds. */
+int ox2d_last_step_ms(ox2d_solver* s, double* ms);
+
+/* Device bytes ox2d_create allocates for this configuration. */
+int ox2d_footprint(const ox2d_config* cfg, uint64_t* bytes);
+
+/* Admits a 2-D case into a batch; like ox_batch_admit.  ox_batch_retire of its
+ * id copies out Q(nx,4,ny). */
+int ox_batch_admit2d(ox_batch* b, const ox2d_config* cfg, const double* Q, int32_t nsteps, int64_t* id);
+
 /* Thread-local text of the last error on this thread ("" if none). */
 const char* ox_last_error(void);
 

@@ -56,6 +56,36 @@ class OxConfig(ctypes.Structure):
     ]
 
 
+OX2D_BC_OBLIQUE_SHOCK = 1
+
+
+class Ox2dConfig(ctypes.Structure):
+    """``ox2d_config`` -- one 2-D case with the oblique-shock boundaries."""
+
+    _fields_ = [
+        ("abi_version", ctypes.c_int32),
+        ("nx", ctypes.c_int32),
+        ("ny", ctypes.c_int32),
+        ("order", ctypes.c_int32),
+        ("scheme", ctypes.c_int32),
+        ("tvd", ctypes.c_int32),
+        ("slau_variant", ctypes.c_int32),
+        ("rk", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("bc", ctypes.c_int32),
+        ("top_split", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("gamma", ctypes.c_double),
+        ("R", ctypes.c_double),
+        ("threshold", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("Lx", ctypes.c_double),
+        ("Ly", ctypes.c_double),
+        ("inflow", ctypes.c_double * 4),
+        ("post", ctypes.c_double * 4),
+    ]
+
+
 class OxDiag(ctypes.Structure):
     """``ox_diag`` -- interior diagnostics."""
 
@@ -141,6 +171,14 @@ EXPORTS = (
     "ox_batch_done",
     "ox_batch_retire",
     "ox_batch_query",
+    "ox2d_create",
+    "ox2d_destroy",
+    "ox2d_set_state",
+    "ox2d_get_state",
+    "ox2d_step",
+    "ox2d_last_step_ms",
+    "ox2d_footprint",
+    "ox_batch_admit2d",
     "ox_last_error",
     "ox_version",
 )

@@ -81,6 +81,20 @@ def load() -> ctypes.CDLL:
     L.ox_batch_retire.restype = I
     L.ox_batch_query.argtypes = [P, ctypes.POINTER(_abi.OxBatchStats)]
     L.ox_batch_query.restype = I
+    L.ox2d_create.argtypes = [ctypes.POINTER(_abi.Ox2dConfig), ctypes.POINTER(P)]
+    L.ox2d_create.restype = I
+    L.ox2d_destroy.argtypes = [P]
+    L.ox2d_destroy.restype = None
+    for f in ("ox2d_set_state", "ox2d_get_state", "ox2d_last_step_ms"):
+        getattr(L, f).argtypes = [P, D]
+        getattr(L, f).restype = I
+    L.ox2d_step.argtypes = [P, ctypes.c_int32]
+    L.ox2d_step.restype = I
+    L.ox2d_footprint.argtypes = [ctypes.POINTER(_abi.Ox2dConfig), ctypes.POINTER(ctypes.c_uint64)]
+    L.ox2d_footprint.restype = I
+    L.ox_batch_admit2d.argtypes = [P, ctypes.POINTER(_abi.Ox2dConfig), D, ctypes.c_int32,
+                                   ctypes.POINTER(ctypes.c_int64)]
+    L.ox_batch_admit2d.restype = I
     L.ox_last_error.argtypes = []
     L.ox_last_error.restype = ctypes.c_char_p
     L.ox_version.argtypes = []

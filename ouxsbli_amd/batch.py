@@ -23,11 +23,14 @@ class Busy(Exception):
     """Admission refused for now (no free slot or HBM budget)."""
 
 
-def footprint(spec: CaseSpec) -> int:
-    """Device bytes a case of this specification holds while resident."""
+def footprint(spec) -> int:
+    """Device bytes a case of this specification (3-D CaseSpec or 2-D
+    CaseSpec2D) holds while resident."""
     b = ctypes.c_uint64()
     cfg = spec.to_ox()
-    _native.check(_native.load().ox_footprint(ctypes.byref(cfg), ctypes.byref(b)))
+    L = _native.load()
+    fn = L.ox2d_footprint if getattr(spec, "dim", 3) == 2 else L.ox_footprint
+    _native.check(fn(ctypes.byref(cfg), ctypes.byref(b)))
     return int(b.value)
 
 
@@ -64,16 +67,19 @@ class Batch:
         except Exception:
             pass
 
-    def admit(self, spec: CaseSpec, Q: np.ndarray, nsteps: int) -> int:
-        """Admits a case (physical state Q, numpy shape (nz, ny, 5, nx)) for
-        nsteps RK steps; returns its id.  Raises Busy if it does not fit now."""
+    def admit(self, spec, Q: np.ndarray, nsteps: int) -> int:
+        """Admits a case for nsteps RK steps and returns its id: a 3-D
+        CaseSpec with physical state Q of numpy shape (nz, ny, 5, nx), or a
+        2-D CaseSpec2D with Q of shape (ny, 4, nx).  Raises Busy if it does
+        not fit now."""
         Q = np.ascontiguousarray(Q, dtype=np.float64)
         if Q.shape != spec.state_shape():
             raise ValueError(f"state shape {Q.shape} != {spec.state_shape()}")
         cfg = spec.to_ox()
         cid = ctypes.c_int64()
-        rc = self._lib.ox_batch_admit(self._h, ctypes.byref(cfg), Q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                      int(nsteps), ctypes.byref(cid))
+        fn = self._lib.ox_batch_admit2d if getattr(spec, "dim", 3) == 2 else self._lib.ox_batch_admit
+        rc = fn(self._h, ctypes.byref(cfg), Q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(nsteps),
+                ctypes.byref(cid))
         if rc == _abi.OX_EBUSY:
             raise Busy(self._lib.ox_last_error().decode())
         _native.check(rc)

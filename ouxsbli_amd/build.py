@@ -38,6 +38,9 @@ UNITS = {
     "ox_fp64_batch.hip": ["-ffp-contract=off"],
     "ox_fp32.hip": ["-ffp-contract=on"],
     "ox_fp32_batch.hip": ["-ffp-contract=on"],
+    "ox2d_capi.hip": ["-ffp-contract=off"],
+    "ox2d_fp64.hip": ["-ffp-contract=off"],
+    "ox2d_fp32.hip": ["-ffp-contract=on"],
 }
 
 

@@ -24,6 +24,7 @@
 #include <tuple>
 #include <vector>
 
+#include "ox2d_core.hpp"
 #include "ox_core.hpp"
 #include "ox_kernels.hpp"
 #include "ox_solver.hpp"
@@ -34,6 +35,10 @@ void ox_destroy(ox_solver* s);
 int ox_set_state(ox_solver* s, const double* Q);
 int ox_get_state(ox_solver* s, double* Q);
 int ox_autotune(ox_solver* s, int32_t* lseg_y, int32_t* lseg_z);
+int ox2d_create(const ox2d_config* cfg, ox2d_solver** out);
+void ox2d_destroy(ox2d_solver* s);
+int ox2d_set_state(ox2d_solver* s, const double* Q);
+int ox2d_get_state(ox2d_solver* s, double* Q);
 }
 
 namespace {
@@ -43,17 +48,30 @@ constexpr size_t kMaxArgs = 2048;
 static_assert(sizeof(ox::SweepArgs<double, double>) <= kMaxArgs, "sweep descriptor size");
 static_assert(sizeof(ox::SweepArgs<float, float>) <= kMaxArgs, "sweep descriptor size");
 
+// A resident case: a 3-D solver (s) or a 2-D one (s2).
 struct BCase {
   int64_t id;
-  ox_solver* s;
+  ox_solver* s = nullptr;
+  ox2d_solver* s2 = nullptr;
   int remaining;
   size_t bytes;
   int group;  // index into ox_batch::groups while active
+
+  int dim() const { return s2 ? 2 : 3; }
+  double cells() const {
+    if (s2) return (double)(s2->cfg.nx - 2) * (s2->cfg.ny - 2);
+    const ox::Geom& g = s->C.g;
+    return (double)g.ix * g.iy * g.iz;
+  }
+  int stages() const { return s2 ? 3 : s->C.cfg.rk; }
 };
 
+// Cases sharing a kernel specialisation.  3-D groups launch three sweeps per
+// stage; 2-D groups one fused stage kernel (launch2[stage]).
 struct BGroup {
-  int prec, key, rk;
+  int dim, prec, key, rk;
   ox::BatchOps ops;
+  ox2::Launch2 launch2[3];
   std::vector<BCase*> members;
 };
 
@@ -117,37 +135,64 @@ int ensure_device(char** p, size_t* cap, size_t need) {
 }
 
 // Groups and the per-step launch plan for the current active cases: tile
-// maps (uploaded here), descriptor offsets, persistent grid sizes.
+// maps (uploaded here), descriptor offsets, persistent grid sizes.  The
+// descriptors of 2-D groups do not change from step to step and are written
+// here; those of 3-D groups are refreshed before every step.
 int build_plan(ox_batch* b) {
   b->groups.clear();
   for (auto& up : b->active) {
     BCase* c = up.get();
-    const ox::Core& C = c->s->C;
-    const int prec = C.cfg.precision, key = ox::stage_key(C.cfg.order, C.cfg.scheme, C.cfg.visc, C.vorder);
+    const int dim = c->dim();
+    int prec, key, rk;
+    if (dim == 3) {
+      const ox::Core& C = c->s->C;
+      prec = C.cfg.precision;
+      key = ox::stage_key(C.cfg.order, C.cfg.scheme, C.cfg.visc, C.vorder);
+      rk = C.cfg.rk;
+    } else {
+      const ox2d_config& k2 = c->s2->cfg;
+      prec = k2.precision;
+      key = ox2::key2(k2.order, k2.scheme);
+      rk = 3;
+    }
     int gi = -1;
-    for (size_t g = 0; g < b->groups.size(); ++g)
-      if (b->groups[g].prec == prec && b->groups[g].key == key && b->groups[g].rk == C.cfg.rk) gi = (int)g;
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+      const BGroup& G = b->groups[g];
+      if (G.dim == dim && G.prec == prec && G.key == key && G.rk == rk) gi = (int)g;
+    }
     if (gi < 0) {
-      BGroup G;
+      BGroup G{};
+      G.dim = dim;
       G.prec = prec;
       G.key = key;
-      G.rk = C.cfg.rk;
-      const bool ok = prec == OX_PREC_FP64 ? ox::find_batch_fp64(C.cfg.order, C.cfg.scheme, C.cfg.visc, C.vorder, &G.ops)
-                                           : ox::find_batch_fp32(C.cfg.order, C.cfg.scheme, C.cfg.visc, C.vorder, &G.ops);
-      if (!ok) return ox_fail(OX_EUNSUPPORTED, "no batched specialisation for a case");
+      G.rk = rk;
+      if (dim == 3) {
+        const ox::Core& C = c->s->C;
+        const bool ok = prec == OX_PREC_FP64 ? ox::find_batch_fp64(C.cfg.order, C.cfg.scheme, C.cfg.visc, C.vorder, &G.ops)
+                                             : ox::find_batch_fp32(C.cfg.order, C.cfg.scheme, C.cfg.visc, C.vorder, &G.ops);
+        if (!ok) return ox_fail(OX_EUNSUPPORTED, "no batched specialisation for a case");
+      } else {
+        for (int st = 0; st < 3; ++st) G.launch2[st] = c->s2->launch[st];
+      }
       b->groups.push_back(G);
       gi = (int)b->groups.size() - 1;
     }
     c->group = gi;
     b->groups[gi].members.push_back(c);
   }
-  // tile maps: per group, per direction, one TileMap per member
+  // tile maps: per 3-D group, per direction, one TileMap per member; 2-D
+  // groups count their tiles (the case lookup reads the descriptors)
   std::vector<ox::TileMap> maps;
   std::vector<size_t> map_off;  // per group: index of its x map (y at +n, z at +2n)
   std::vector<std::array<int, 3>> ntiles;
   for (BGroup& G : b->groups) {
     map_off.push_back(maps.size());
     std::array<int, 3> nt{0, 0, 0};
+    if (G.dim == 2) {
+      for (BCase* c : G.members) nt[0] += c->s2->ntiles;
+      ntiles.push_back(nt);
+      continue;
+    }
     for (int d = 0; d < 3; ++d) {
       for (BCase* c : G.members) {
         const ox::Geom& g = c->s->C.g;
@@ -190,16 +235,21 @@ int build_plan(ox_batch* b) {
       ox::BatchGroupLaunch L{};
       L.stream = b->stream;
       L.ncase = n;
-      for (int d = 0; d < 3; ++d) {
-        L.map[d] = b->d_map + map_off[gi] + (size_t)d * n;
-        L.ntiles[d] = ntiles[gi][d];
-        L.grid[d] = std::max(1, std::min(ntiles[gi][d], kMaxGrid));
+      if (G.dim == 3) {
+        for (int d = 0; d < 3; ++d) {
+          L.map[d] = b->d_map + map_off[gi] + (size_t)d * n;
+          L.ntiles[d] = ntiles[gi][d];
+          L.grid[d] = std::max(1, std::min(ntiles[gi][d], kMaxGrid));
+        }
+      } else {
+        L.ntiles[0] = ntiles[gi][0];
+        L.grid[0] = ox2::grid2(ntiles[gi][0], G.launch2[st].max_blocks());
       }
       b->launches.push_back(L);
       b->launch_stage.push_back(st);
       b->launch_grp.push_back((int)gi);
       b->args_off.push_back(off);
-      off += 3 * (size_t)n * G.ops.args_bytes;
+      off += G.dim == 3 ? 3 * (size_t)n * G.ops.args_bytes : (size_t)n * ox2::args2_bytes(G.prec);
       off = (off + 255) & ~(size_t)255;
     }
   }
@@ -217,8 +267,19 @@ int build_plan(ox_batch* b) {
   for (size_t li = 0; li < b->launches.size(); ++li) {
     const BGroup& G = b->groups[b->launch_grp[li]];
     const size_t n = G.members.size();
-    for (int d = 0; d < 3; ++d)
-      b->launches[li].args[d] = b->d_args + b->args_off[li] + (size_t)d * n * G.ops.args_bytes;
+    if (G.dim == 3) {
+      for (int d = 0; d < 3; ++d)
+        b->launches[li].args[d] = b->d_args + b->args_off[li] + (size_t)d * n * G.ops.args_bytes;
+    } else {
+      b->launches[li].args[0] = b->d_args + b->args_off[li];
+      const size_t ab = ox2::args2_bytes(G.prec);
+      int tile0 = 0;
+      for (size_t m = 0; m < n; ++m) {
+        ox2d_solver* s2 = G.members[m]->s2;
+        ox2::write_args2(s2->cfg, s2->g, s2->buf, b->launch_stage[li], tile0, b->h_args + b->args_off[li] + m * ab);
+        tile0 += s2->ntiles;
+      }
+    }
     b->launches[li].ep = 0;  // set per step
   }
   b->plan_valid = true;
@@ -243,7 +304,23 @@ int tune_case(ox_batch* b, ox_solver* s) {
 }
 
 void enqueue_step(ox_batch* b) {
-  for (size_t li = 0; li < b->launches.size(); ++li) b->groups[b->launch_grp[li]].ops.launch(b->launches[li]);
+  for (size_t li = 0; li < b->launches.size(); ++li) {
+    const BGroup& G = b->groups[b->launch_grp[li]];
+    const ox::BatchGroupLaunch& L = b->launches[li];
+    if (G.dim == 3) G.ops.launch(L);
+    else G.launch2[b->launch_stage[li]].launch(L.args[0], L.ncase, L.ntiles[0], L.grid[0], L.stream);
+  }
+}
+
+// Returns a case's solver to its own stream and frees it.
+void free_case(BCase* c) {
+  if (c->s2) {
+    c->s2->stream = c->s2->own_stream;
+    ox2d_destroy(c->s2);
+  } else {
+    c->s->C.stream = c->s->own_stream;
+    ox_destroy(c->s);
+  }
 }
 
 }  // namespace
@@ -326,10 +403,6 @@ void ox_batch_destroy(ox_batch* b) {
   hipSetDevice(b->cfg.device);
   if (b->stream) hipStreamSynchronize(b->stream);
   drop_graphs(b);
-  auto free_case = [](BCase* c) {
-    c->s->C.stream = c->s->own_stream;
-    ox_destroy(c->s);
-  };
   for (auto& c : b->active) free_case(c.get());
   for (auto& c : b->done) free_case(c.get());
   if (b->d_args) hipFree(b->d_args);
@@ -401,6 +474,35 @@ int ox_batch_tune(ox_batch* b, const ox_config* cfg, const double* Q) {
   return rc;
 }
 
+int ox_batch_admit2d(ox_batch* b, const ox2d_config* cfg, const double* Q, int32_t nsteps, int64_t* id) {
+  if (!b || !cfg || !Q || !id) return ox_fail(OX_EINVAL, "null argument");
+  if (nsteps < 1) return ox_fail(OX_EINVAL, "nsteps must be >= 1");
+  if (cfg->device != b->cfg.device) return ox_fail(OX_EINVAL, "case device differs from the batch device");
+  const size_t need = solver2d_footprint(*cfg);
+  if ((int)(b->active.size() + b->done.size()) >= b->cfg.max_cases) return ox_fail(OX_EBUSY, "no free slot");
+  if (b->bytes_in_use + need > b->cfg.hbm_budget) return ox_fail(OX_EBUSY, "footprint exceeds the free HBM budget");
+  ox2d_solver* s = nullptr;
+  int rc = ox2d_create(cfg, &s);
+  if (rc != OX_OK) return rc;
+  rc = ox2d_set_state(s, Q);
+  if (rc != OX_OK) {
+    ox2d_destroy(s);
+    return rc;
+  }
+  s->stream = b->stream;
+  auto c = std::make_unique<BCase>();
+  c->id = b->next_id++;
+  c->s2 = s;
+  c->remaining = nsteps;
+  c->bytes = need;
+  c->group = -1;
+  b->bytes_in_use += need;
+  *id = c->id;
+  b->active.push_back(std::move(c));
+  invalidate(b);
+  return OX_OK;
+}
+
 int ox_batch_step(ox_batch* b, int32_t* active) {
   if (!b) return ox_fail(OX_EINVAL, "null handle");
   OX_HIP(hipSetDevice(b->cfg.device));
@@ -416,6 +518,7 @@ int ox_batch_step(ox_batch* b, int32_t* active) {
   // advances stage by stage exactly as in ox_step
   for (size_t li = 0; li < b->launches.size(); ++li) {
     BGroup& G = b->groups[b->launch_grp[li]];
+    if (G.dim == 2) continue;  // written with the plan
     const int st = b->launch_stage[li];
     const size_t n = G.members.size(), ab = G.ops.args_bytes;
     char* base = b->h_args + b->args_off[li];
@@ -468,10 +571,10 @@ int ox_batch_step(ox_batch* b, int32_t* active) {
   bool changed = false;
   for (size_t a = 0; a < b->active.size();) {
     BCase* c = b->active[a].get();
-    const ox::Geom& g = c->s->C.g;
-    b->cell_stages += (double)g.ix * g.iy * g.iz * c->s->C.cfg.rk;
+    b->cell_stages += c->cells() * c->stages();
     if (--c->remaining == 0) {
-      ox::launch_ghost_fill(c->s->C, c->s->QJ);  // the result is read with its ghost layers
+      // the result is read with its ghost layers (2-D: written by the last stage)
+      if (c->s) ox::launch_ghost_fill(c->s->C, c->s->QJ);
       b->done.push_back(std::move(b->active[a]));
       b->active.erase(b->active.begin() + a);
       changed = true;
@@ -506,9 +609,8 @@ int ox_batch_retire(ox_batch* b, int64_t id, double* Q) {
     if ((*it)->id != id) continue;
     BCase* c = it->get();
     int rc = OX_OK;
-    if (Q) rc = ox_get_state(c->s, Q);
-    c->s->C.stream = c->s->own_stream;
-    ox_destroy(c->s);
+    if (Q) rc = c->s2 ? ox2d_get_state(c->s2, Q) : ox_get_state(c->s, Q);
+    free_case(c);
     b->bytes_in_use -= c->bytes;
     b->done.erase(it);
     return rc;
@@ -523,11 +625,16 @@ int ox_batch_query(ox_batch* b, ox_batch_stats* out) {
   out->done = (int32_t)b->done.size();
   int ng = 0;
   {
-    std::vector<std::tuple<int, int, int>> seen;
+    std::vector<std::tuple<int, int, int, int>> seen;
     for (auto& c : b->active) {
-      const ox::Core& C = c->s->C;
-      auto t = std::make_tuple((int)C.cfg.precision, ox::stage_key(C.cfg.order, C.cfg.scheme, C.cfg.visc, C.vorder),
-                               (int)C.cfg.rk);
+      std::tuple<int, int, int, int> t;
+      if (c->s2) {
+        t = std::make_tuple(2, (int)c->s2->cfg.precision, ox2::key2(c->s2->cfg.order, c->s2->cfg.scheme), 3);
+      } else {
+        const ox::Core& C = c->s->C;
+        t = std::make_tuple(3, (int)C.cfg.precision, ox::stage_key(C.cfg.order, C.cfg.scheme, C.cfg.visc, C.vorder),
+                            (int)C.cfg.rk);
+      }
       if (std::find(seen.begin(), seen.end(), t) == seen.end()) seen.push_back(t);
     }
     ng = (int)seen.size();

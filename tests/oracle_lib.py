@@ -46,6 +46,17 @@ def lib():
                                 D, D]
         L.oxo_wiggle.argtypes = [D]
         L.oxo_wiggle.restype = ctypes.c_double
+        L.oxo2d_create.restype = P
+        L.oxo2d_create.argtypes = [ctypes.POINTER(_abi.Ox2dConfig)]
+        L.oxo2d_destroy.argtypes = [P]
+        for f in ("oxo2d_set_state", "oxo2d_get_state"):
+            getattr(L, f).argtypes = [P, D]
+            getattr(L, f).restype = ctypes.c_int
+        L.oxo2d_step.argtypes = [P, ctypes.c_int]
+        L.oxo2d_step.restype = ctypes.c_int
+        L.oxo2d_last_error.restype = ctypes.c_char_p
+        L.oxo2d_face_flux.argtypes = [ctypes.POINTER(_abi.Ox2dConfig), ctypes.c_int, ctypes.c_int, ctypes.c_int, D,
+                                      ctypes.c_double, D]
         _lib = L
     return _lib
 
@@ -112,3 +123,45 @@ class Oracle:
         d = _abi.OxDiag()
         lib().oxo_diagnostics(self.h, ctypes.byref(d))
         return d.as_dict()
+
+
+class Oracle2D:
+    """The 2-D oblique-shock transcription driven through its C entry points."""
+
+    def __init__(self, spec):
+        self.spec = spec
+        self.cfg = spec.to_ox()
+        self.h = lib().oxo2d_create(ctypes.byref(self.cfg))
+        if not self.h:
+            raise RuntimeError(lib().oxo2d_last_error().decode())
+
+    def close(self):
+        if self.h:
+            lib().oxo2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_state(self, Q):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        assert Q.shape == self.spec.state_shape()
+        lib().oxo2d_set_state(self.h, _dp(Q))
+
+    def get_state(self):
+        Q = np.empty(self.spec.state_shape())
+        lib().oxo2d_get_state(self.h, _dp(Q))
+        return Q
+
+    def step(self, n=1):
+        lib().oxo2d_step(self.h, int(n))
+
+
+def face_flux2d(spec, direction, f, n, st, fs):
+    """Oracle flux of one 2-D face (see ox_oracle2d.cpp, Oracle2D::face):
+    st[4][6] primitives (rho, u, v, p) at points f-2..f+3."""
+    cfg = spec.to_ox()
+    st = np.ascontiguousarray(st, dtype=np.float64).reshape(4, 6)
+    out = np.empty(4)
+    lib().oxo2d_face_flux(ctypes.byref(cfg), int(direction), int(f), int(n), _dp(st), float(fs), _dp(out))
+    return out

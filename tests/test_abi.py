@@ -9,7 +9,7 @@ from conftest import ROOT
 
 def _declared():
     text = (ROOT / "include" / "ouxsbli.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(ox_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(ox(?:2d)?_\w+)\s*\(", text, re.M)))
 
 
 def test_header_and_python_agree():

@@ -141,6 +141,88 @@ def run_form_state(spec, Q, steps, warmup):
     return {"wall_s": wall, "diag": diag, "ok": ok}
 
 
+OS_BYTES_PER_CELL_STAGE = (64.0 + 96.0 + 96.0) / 3.0  # 2-D RK3: stage 1 reads Q, writes Q1; stages 2, 3 read two
+
+
+def batched_os_leg(ncase, steps, unbatched_steps, cpu_seconds):
+    """Batched ensemble of 2-D oblique-shock solves: 64 cases, interior grids
+    nx-2 ~ U{128..512}, ny-2 ~ U{64..256}, Mach 2, deflection ~ U(4, 12) deg
+    (seeded), Rankine-Hugoniot state on the top boundary, Roe + MUSCL (ORDER 6,
+    hybrid limiter), RK3, FP32 bulk, `steps` steps each, all resident and
+    advanced together: one persistent launch per RK stage over the tiles of
+    all cases, replayed from a HIP graph.  Timed beside: the FP64 plain form
+    of the same batch, and the FP32 cases advanced one by one (one launch per
+    stage per case)."""
+    import numpy as np
+    from ouxsbli_amd import cases2d as c2
+    from ouxsbli_amd.batch import Batch
+    from ouxsbli_amd.solver2d import Solver2D
+    out = {"workload": f"{ncase} batched 2-D oblique-shock reflections, interior 128..512 x 64..256, M 2, "
+                       "theta U(4,12) deg, Roe + MUSCL ORDER 6 hybrid limiter, RK3, "
+                       f"{steps} steps each, one persistent launch per stage with HIP graph replay",
+           "cases": ncase, "steps": steps}
+    res = {}
+    for prec in ("fp32_bulk", "fp64"):
+        specs = c2.config4_specs(ncase, precision=prec)
+        states = [c2.os_state(s) for s in specs]
+        cs = sum(s.interior_cells for s in specs) * 3 * steps
+        with Batch(hbm_budget=32 << 30, max_cases=ncase, use_graph=True, autotune=False) as b:
+            ids = [b.admit(s, Q, steps) for s, Q in zip(specs, states)]
+            t0 = time.perf_counter()
+            while b.step():
+                pass
+            wall = time.perf_counter() - t0
+            st = b.stats()
+            final = {cid: b.retire(cid) for cid in ids}
+        res[prec] = [final[cid] for cid in ids]
+        dev = st["device_ms"] * 1e-3
+        out[prec] = {"Mcell_updates_per_s_stage": round(cs / wall / 1e6, 1),
+                     "device_busy_Mcell_updates_per_s_stage": round(cs / dev / 1e6, 1),
+                     "wall_s": round(wall, 3), "graph_replays": st["graph_replays"],
+                     "avg_stage_launch_ms": round(st["device_ms"] / (3 * steps), 4),
+                     "alg_GBs": round(cs / dev * OS_BYTES_PER_CELL_STAGE / 1e9, 1)}
+        out["cell_stages"] = cs
+    out["fp32_bulk"]["speedup_vs_fp64"] = round(out["fp32_bulk"]["Mcell_updates_per_s_stage"] /
+                                                out["fp64"]["Mcell_updates_per_s_stage"], 3)
+    # accuracy of the FP32-bulk ensemble against the FP64 one, per case
+    errs = []
+    for a, b in zip(res["fp64"], res["fp32_bulk"]):
+        d = a[:, 0, :] - b[:, 0, :]
+        errs.append(float(np.sqrt(np.mean(d * d) / np.mean(a[:, 0, :] ** 2))))
+    out["fp32_bulk"]["rho_rel_l2_vs_fp64_max"] = max(errs)
+    out["fp32_bulk"]["rho_rel_l2_vs_fp64_median"] = float(np.median(errs))
+    # the same FP32 cases one at a time (one launch per stage per case)
+    specs = c2.config4_specs(ncase, precision="fp32_bulk")
+    t_dev = 0.0
+    cs1 = 0
+    for s in specs:
+        with Solver2D(s) as h:
+            h.set_state(c2.os_state(s))
+            h.step(unbatched_steps)
+            t_dev += h.last_step_ms() * 1e-3
+        cs1 += s.interior_cells * 3 * unbatched_steps
+    out["fp32_bulk_one_case_at_a_time"] = {"device_Mcell_updates_per_s_stage": round(cs1 / t_dev / 1e6, 1),
+                                           "steps": unbatched_steps}
+    if cpu_seconds > 0:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib
+        s = c2.os_spec(258, 130, scheme="Roe", cfl=c2.CFL_REF)
+        o = oracle_lib.Oracle2D(s)
+        o.set_state(c2.os_state(s))
+        o.step(1)
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < cpu_seconds:
+            o.step(10)
+            n += 10
+        el = time.perf_counter() - t0
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        out["cpu_oracle"] = {"value": round(s.interior_cells * 3 * n / el / 1e6, 1),
+                             "unit": "Mcell-updates/s per RK stage", "cores": cores, "kind": "port",
+                             "sample": f"2-D oracle (FP64 OpenMP), one 256 x 128 Roe case, {n} steps ({el:.1f} s)"}
+    return out
+
+
 SERVE_BUDGET = 64 << 30
 SERVE_SLOTS = 16
 
@@ -198,6 +280,11 @@ def main():
     ap.add_argument("--les-n", type=int, default=384,
                     help="interior size of the LES decaying-turbulence leg (0: skip it)")
     ap.add_argument("--les-steps", type=int, default=3)
+    ap.add_argument("--os-cases", type=int, default=64,
+                    help="cases of the batched 2-D oblique-shock leg (0: skip it)")
+    ap.add_argument("--os-steps", type=int, default=2000)
+    ap.add_argument("--os-only", action="store_true",
+                    help="run only the batched oblique-shock leg and print its JSON (for profiling)")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the Kelvin-Helmholtz serving stream (0: skip the serving leg)")
     args = ap.parse_args()
@@ -205,6 +292,10 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no multi-GPU path; "
               "the z-slab sharding design is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
+
+    if args.os_only:
+        print(json.dumps(batched_os_leg(args.os_cases, args.os_steps, 100, 0.0)), flush=True)
+        return
 
     from ouxsbli_amd import cases
     spec = cases.config2_spec(args.n, "fp64")
@@ -276,6 +367,9 @@ def main():
                                    "form": "FP64 plain, the same workload with the ME4Base viscous stencil"}
     if args.les_n > 0:
         line["les_dhit"] = les_leg(args.les_n, args.les_steps)
+    if args.os_cases > 0:
+        line["batched_oblique_shock"] = batched_os_leg(args.os_cases, args.os_steps, 100,
+                                                       0.0 if args.no_cpu else 5.0)
     if args.serve_requests > 0:
         line["serving"] = serving_leg(args.serve_requests, value)
     if not args.no_cpu:

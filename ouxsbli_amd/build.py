@@ -6,8 +6,10 @@ The FP64 units are compiled with -ffp-contract=off (every multiply and add
 rounded separately, in the evaluation order of the reference expressions);
 the FP32-bulk units contract a*b+c within one source expression
 (-ffp-contract=on), a rule that does not depend on the surrounding code, so
-the single-case and the batched launches of a case round identically.  Packed FP32 instructions are disabled
-for all device code.
+the single-case and the batched launches of a case round identically, and
+take FP32 division and square root in their fast hardware forms (about 1 ulp
+instead of correctly rounded).  Packed FP32 instructions are disabled for all
+device code.
 """
 from __future__ import annotations
 
@@ -36,11 +38,11 @@ UNITS = {
     "ox_batch.hip": ["-ffp-contract=off"],
     "ox_fp64.hip": ["-ffp-contract=off"],
     "ox_fp64_batch.hip": ["-ffp-contract=off"],
-    "ox_fp32.hip": ["-ffp-contract=on"],
-    "ox_fp32_batch.hip": ["-ffp-contract=on"],
+    "ox_fp32.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "ox_fp32_batch.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "ox2d_capi.hip": ["-ffp-contract=off"],
     "ox2d_fp64.hip": ["-ffp-contract=off"],
-    "ox2d_fp32.hip": ["-ffp-contract=on"],
+    "ox2d_fp32.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }
 
 

@@ -64,8 +64,16 @@ template <class R> struct Args2 {
 };
 
 constexpr int TX = 64;
-// rows per tile: keeps the workgroup's LDS under 64 KiB
-template <class R> constexpr int tile_rows() { return sizeof(R) == 8 ? 6 : 15; }
+// Rows per tile.  11 FP32 rows: 715 x faces, 768 y faces and 704 cells
+// keep the 256 threads busy in three passes each, and 46 KiB of LDS lets
+// three workgroups share a CU.  4 FP64 rows: 44 KiB, three per CU.
+#ifndef OX2_ROWS_FP64
+#define OX2_ROWS_FP64 4
+#endif
+#ifndef OX2_ROWS_FP32
+#define OX2_ROWS_FP32 11
+#endif
+template <class R> constexpr int tile_rows() { return sizeof(R) == 8 ? OX2_ROWS_FP64 : OX2_ROWS_FP32; }
 // halo of the LDS tile: the stencil half-width, at least 2 where the sensor
 // (2nd-order central differences at the points next to the faces) is needed
 template <int ORDER> constexpr int halo() { return ORDER == 6 ? 3 : (ORDER == 4 ? 2 : 1); }

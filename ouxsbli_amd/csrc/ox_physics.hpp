@@ -148,13 +148,22 @@ OX_HD void keep_face(const PhysC<R>& c, const R* rho, const R* u, const R* v, co
 // ---------------------------------------------------------------------------
 // MUSCL.  src/calc_muscl.f90:19-169.  tvd: OX_TVD_NONE / TVD / HYBRID.
 // ---------------------------------------------------------------------------
+// MIN/MAX of the limiters: the select form in FP64, where the results must
+// match the FP64 oracle bit for bit, and the hardware min/max (one
+// instruction, no compare-and-select) in FP32, which differs from it only in
+// the sign of a zero result -- and a limiter's zero slope adds nothing either way.
+template <class R> OX_HD R lmin(R a, R b) { return fmin2(a, b); }
+template <class R> OX_HD R lmax(R a, R b) { return fmax2(a, b); }
+template <> OX_HD float lmin<float>(float a, float b) { return __builtin_fminf(a, b); }
+template <> OX_HD float lmax<float>(float a, float b) { return __builtin_fmaxf(a, b); }
+
 template <class R> OX_HD R minmod2(R x, R y) {
   const R sgn = fsign1(x);
-  return sgn * fmax2(fmin2(fabs_(x), sgn * y), R(0));
+  return sgn * lmax(lmin(fabs_(x), sgn * y), R(0));
 }
 template <class R> OX_HD R minmod3(R x, R y, R z) {
   const R sgn = fsign1(x);
-  return sgn * fmax2(fmin2(fmin2(fabs_(x), sgn * y), sgn * z), R(0));
+  return sgn * lmax(lmin(lmin(fabs_(x), sgn * y), sgn * z), R(0));
 }
 template <class R> OX_HD R d33(R d1, R d2, R d3) {
   return minmod3(d1, R(2) * d2, R(2) * d3) - R(2) * minmod3(d2, R(2) * d1, R(2) * d3) +

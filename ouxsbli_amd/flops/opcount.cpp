@@ -12,7 +12,11 @@
 //   + 3 x (convective face + viscous face)
 //   + the 9 distinct velocity cross derivatives of the viscous terms
 //   + calc_R (3 x 5 x (sub, mul, add) minus the first add) + the RK combination.
+// 2-D oblique-shock path (ox2d_kernels.hpp): per cell and stage, primitive
+// recovery + Ducros sensor + 2 x (MUSCL on 4 primitives + face flux + face
+// sensor average) + calc_R (2 x 4 x (sub, mul) + 4 adds) + the RK combination.
 // Output: one JSON object per configuration on stdout.
+// Build: g++ -std=c++17 -O1 -I/opt/rocm/include -D__HIP_PLATFORM_AMD__ -x c++ opcount.cpp
 #include <cstdio>
 #include <cstring>
 
@@ -116,10 +120,49 @@ static void report(const char* name, int order, int scheme, int visc, int vo, in
               name, order, scheme, visc, vo, rk, conv_smooth, conv_tvd, vf, cd, smooth, limited);
 }
 
+// One x face of the 2-D path: MUSCL (6-point) on rho, u, v, p, then the
+// flux of the 3-D routine with w = 0 (ox2d_kernels.hpp, flux2).
+static long long face2d_ops(int scheme, bool limited) {
+  PhysC<Cnt> c = consts(0.4);
+  Cnt st[4][6], L[4], Rr[4], F5[5];
+  fill(st[0], 6, 1.0, 0.01, limited);
+  fill(st[1], 6, 300.0, 2.0, false);
+  fill(st[2], 6, -20.0, 1.0, false);
+  fill(st[3], 6, 1.0e5, 1.0e2, limited);
+  g_ops = 0;
+  for (int v = 0; v < 4; ++v) {
+    if (limited) delta6_form<Cnt, true>(c, st[v], L[v], Rr[v]);
+    else delta6_form<Cnt, false>(c, st[v], L[v], Rr[v]);
+  }
+  if (scheme == 2) {
+    roe_local<Cnt>(c, L[0], Rr[0], L[1], Rr[1], L[2], Rr[2], Cnt(0.0), Cnt(0.0), L[3], Rr[3], F5);
+  } else {
+    const Cnt hr = wiggle<Cnt>(&st[3][1]);
+    slau<Cnt, 0>(c, 1, L[0], Rr[0], L[1], Rr[1], L[2], Rr[2], Cnt(0.0), Cnt(0.0), L[3], Rr[3], hr, F5);
+  }
+  return g_ops;
+}
+
+static void report2d(const char* name, int scheme) {
+  const long long smooth = face2d_ops(scheme, false), limited = face2d_ops(scheme, true);
+  const long long prim = 1 /*1/q0*/ + 1 /*rho*/ + 2 /*u, v*/ + 8 /*p*/;
+  const long long sensor = 4 * 2 /*differences x metric*/ + 2 /*div, rot*/ + 6 /*fd*/;
+  const long long face_avg = 2;
+  const long long calcR = 4 * 5;
+  const long long rk_upd = 4 * 4;
+  const long long base = prim + sensor + calcR + rk_upd;
+  std::printf("{\"config\": \"%s\", \"order\": 6, \"scheme\": %d, \"visc\": 0, \"visc_order\": 0, \"rk\": 3, "
+              "\"dim\": 2, \"conv_face_smooth\": %lld, \"conv_face_limited\": %lld, \"per_cell_stage_smooth\": %lld, "
+              "\"per_cell_stage_limited\": %lld}\n",
+              name, scheme, smooth, limited, base + 2 * (smooth + face_avg), base + 2 * (limited + face_avg));
+}
+
 int main() {
   report("keep4_ns4_rk4", 4, 0, 1, 4, 4, 0, 1);
   report("slau6_hrslau2_hybrid_ns6_rk3", 6, 1, 1, 6, 3, 2, 1);
   report("keep6_ns6_rk3", 6, 0, 1, 6, 3, 0, 1);
   report("slau6_euler_rk3", 6, 1, 0, 0, 3, 2, 1);
+  report2d("roe6_hybrid_2d_rk3", 2);
+  report2d("slau6_hrslau2_hybrid_2d_rk3", 1);
   return 0;
 }

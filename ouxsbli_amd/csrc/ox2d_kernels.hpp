@@ -17,9 +17,11 @@
 //      read (boundary points take their inner neighbour's value);
 //   3. the x faces and y faces of the tile into LDS, each with the order its
 //      distance to the boundary allows (6-, 4- or 2-point);
-//   4. per cell: R = cx dE + cy dF in FP64, the RK combination in FP64, the
-//      new state and -- for cells next to the boundary -- the boundary
-//      points of the oblique-shock conditions, from the new values.
+//   4. per cell: R = cx dE + cy dF in FP64 (the x part as soon as the x
+//      faces are done, kept in registers while the y faces reuse their
+//      LDS), the RK combination in FP64, the new state and -- for cells next
+//      to the boundary -- the boundary points of the oblique-shock
+//      conditions, from the new values.
 //
 // Stages read one state buffer and write another, so tiles never race on
 // halo points (three buffers per case for RK3).  The FP64 form follows the
@@ -65,8 +67,9 @@ template <class R> struct Args2 {
 
 constexpr int TX = 64;
 // Rows per tile.  11 FP32 rows: 715 x faces, 768 y faces and 704 cells
-// keep the 256 threads busy in three passes each, and 46 KiB of LDS lets
-// three workgroups share a CU.  4 FP64 rows: 44 KiB, three per CU.
+// keep the 256 threads busy in three passes each, and 34 KiB of LDS (106
+// VGPRs) lets four workgroups share a CU.  4 FP64 rows: 35 KiB, three per
+// CU (130 VGPRs).
 #ifndef OX2_ROWS_FP64
 #define OX2_ROWS_FP64 4
 #endif
@@ -84,8 +87,13 @@ template <class R, int ORDER> struct Tile2 {
   static constexpr int W = TX + 2 * HL, H = TY + 2 * HL;
   R prim[4][H][W];               // rho, u, v, p
   R sens[TY + 2][TX + 2];        // Ducros sensor, points x0-1..x0+TX, y0-1..y0+TY
-  R E[4][TY][TX + 1];            // x faces x0-1..x0+TX-1 of the tile rows
-  R F[4][TY + 1][TX];            // y faces y0-1..y0+TY-1 of the tile columns
+  // face fluxes: first the x faces x0-1..x0+TX-1 of the tile rows, then --
+  // once each cell holds its x differences in registers -- the y faces
+  // y0-1..y0+TY-1 of the tile columns in the same storage
+  union {
+    R E[4][TY][TX + 1];
+    R F[4][TY + 1][TX];
+  } fl;
 };
 
 // The Ducros sensor of one interior point from the velocity neighbours
@@ -255,8 +263,25 @@ __device__ __forceinline__ void stage_tile(const Args2<R>& a, int tx, int ty, Ti
     R Fl[4];
     face2<R, ORDER, SCHEME, 0>(a, f + 1, nx, st, fs, Fl);
 #pragma unroll
-    for (int l = 0; l < 4; ++l) T.E[l][fy][fx] = Fl[l];
+    for (int l = 0; l < 4; ++l) T.fl.E[l][fy][fx] = Fl[l];
   }
+  __syncthreads();
+
+  // 4a. x part of the residual, R = cx * (-E_i + E_i+1) (calc_R,
+  //     calc_steps.f90:17-21), in registers: cell k = tid + 256 m
+  constexpr int NC = (TX * TY + 255) / 256;
+  double Rc[NC][4];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    const int k = tid + 256 * m;
+    const int rx = k % TX, ry = k / TX;
+    if (k < TX * TY && x0 + rx <= xlast && y0 + ry <= ylast) {
+#pragma unroll
+      for (int l = 0; l < 4; ++l) Rc[m][l] = a.cx * (-(double)T.fl.E[l][ry][rx] + (double)T.fl.E[l][ry][rx + 1]);
+    }
+  }
+  __syncthreads();
+
   // 3b. y faces g = y0-1..ylast of columns x0..xlast
   for (int k = tid; k < TX * (TY + 1); k += 256) {
     const int gx = k % TX, gy = k / TX;
@@ -274,21 +299,21 @@ __device__ __forceinline__ void stage_tile(const Args2<R>& a, int tx, int ty, Ti
     R Fl[4];
     face2<R, ORDER, SCHEME, 1>(a, g + 1, ny, st, fs, Fl);
 #pragma unroll
-    for (int l = 0; l < 4; ++l) T.F[l][gy][gx] = Fl[l];
+    for (int l = 0; l < 4; ++l) T.fl.F[l][gy][gx] = Fl[l];
   }
   __syncthreads();
 
-  // 4. residual (calc_R, calc_steps.f90:8-28), RK combination, boundary points
-  for (int k = tid; k < TX * TY; k += 256) {
+  // 4b. y part of the residual (calc_steps.f90:23-27), RK combination,
+  //     boundary points
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    const int k = tid + 256 * m;
     const int rx = k % TX, ry = k / TX;
     const int i = x0 + rx, j = y0 + ry;
-    if (i > xlast || j > ylast) continue;
+    if (k >= TX * TY || i > xlast || j > ylast) continue;
     double Rr[4];
 #pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      Rr[l] = a.cx * (-(double)T.E[l][ry][rx] + (double)T.E[l][ry][rx + 1]);
-      Rr[l] = Rr[l] + a.cy * (-(double)T.F[l][ry][rx] + (double)T.F[l][ry + 1][rx]);
-    }
+    for (int l = 0; l < 4; ++l) Rr[l] = Rc[m][l] + a.cy * (-(double)T.fl.F[l][ry][rx] + (double)T.fl.F[l][ry + 1][rx]);
     double qs[4], qn[4];
     load_q(a.Qs, nx, ny, i, j, qs);
     if constexpr (EP == EP2_S1) {

@@ -169,7 +169,7 @@ def batched_os_leg(ncase, steps, unbatched_steps, cpu_seconds):
         with Batch(hbm_budget=32 << 30, max_cases=ncase, use_graph=True, autotune=False) as b:
             ids = [b.admit(s, Q, steps) for s, Q in zip(specs, states)]
             t0 = time.perf_counter()
-            while b.step():
+            while b.run(steps)[1]:
                 pass
             wall = time.perf_counter() - t0
             st = b.stats()

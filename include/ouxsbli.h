@@ -246,6 +246,12 @@ int ox_batch_tune(ox_batch* b, const ox_config* cfg, const double* Q);
  * *active receives the number of cases still stepping. */
 int ox_batch_step(ox_batch* b, int32_t* active);
 
+/* Like ox_batch_step for up to max_steps steps, enqueued back to back on the
+ * device with one synchronization at the end.  Stops early after the first
+ * step in which a case reaches its step count (so that it can be retired and
+ * others admitted); *steps_done receives the number of steps run. */
+int ox_batch_run(ox_batch* b, int32_t max_steps, int32_t* steps_done, int32_t* active);
+
 /* Ids of finished cases not yet retired (at most cap, oldest first). */
 int ox_batch_done(ox_batch* b, int64_t* ids, int32_t cap, int32_t* n);
 

@@ -168,6 +168,7 @@ EXPORTS = (
     "ox_batch_admit",
     "ox_batch_tune",
     "ox_batch_step",
+    "ox_batch_run",
     "ox_batch_done",
     "ox_batch_retire",
     "ox_batch_query",

@@ -75,6 +75,8 @@ def load() -> ctypes.CDLL:
     L.ox_batch_tune.restype = I
     L.ox_batch_step.argtypes = [P, ctypes.POINTER(ctypes.c_int32)]
     L.ox_batch_step.restype = I
+    L.ox_batch_run.argtypes = [P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+    L.ox_batch_run.restype = I
     L.ox_batch_done.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
     L.ox_batch_done.restype = I
     L.ox_batch_retire.argtypes = [P, ctypes.c_int64, D]

@@ -100,6 +100,15 @@ class Batch:
         _native.check(self._lib.ox_batch_step(self._h, ctypes.byref(n)))
         return n.value
 
+    def run(self, max_steps: int):
+        """Up to max_steps steps back to back with one synchronization; stops
+        after the first step in which a case finishes.  Returns (steps run,
+        cases still stepping)."""
+        k = ctypes.c_int32()
+        n = ctypes.c_int32()
+        _native.check(self._lib.ox_batch_run(self._h, int(max_steps), ctypes.byref(k), ctypes.byref(n)))
+        return k.value, n.value
+
     def done(self) -> list:
         cap = 1024
         ids = (ctypes.c_int64 * cap)()

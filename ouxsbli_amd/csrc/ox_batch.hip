@@ -4,11 +4,14 @@
 // Every admitted case is an ox_solver whose kernels run on the engine's
 // stream.  Cases with the same kernel specialisation, precision and RK scheme
 // form a group; one RK stage of a group is three persistent launches (x, y, z
-// sweeps) over the concatenated tiles of its cases, each case's tiles reading
-// that case's descriptor (SweepArgs).  The descriptors of a whole step are
-// built on the host (each case's primitive-set parity advances stage by
-// stage), copied to the device in one transfer, and the launches of the step
-// are replayed from a HIP graph captured for the current set of cases.
+// sweeps; one fused launch for 2-D groups) over the concatenated tiles of its
+// cases, each case's tiles reading that case's descriptor.  A 3-D case's
+// descriptors depend on its primitive-set parity, which flips every stage,
+// so they repeat with period two steps (one for RK4): the engine builds the
+// descriptors of the next two steps once per set of cases, keeps both sets
+// on the device and captures one HIP graph per set.  Steps then alternate
+// between the two graphs with no host work in between, and ox_batch_run
+// enqueues many steps back to back before it synchronizes.
 // A case's arithmetic is the same code on the same data as its single-case
 // launches, so its result does not depend on what else is in the batch.
 #include <hip/hip_runtime.h>
@@ -94,16 +97,16 @@ struct ox_batch {
   std::vector<int> launch_stage;
   std::vector<int> launch_grp;
   std::vector<size_t> args_off;                // per launch: byte offset of its x,y,z descriptor arrays
-  size_t args_bytes = 0;
-  char* d_args = nullptr;
+  size_t args_bytes = 0;     // one descriptor set (all launches of one step)
+  char* d_args = nullptr;    // two sets: set k at d_args + k * args_bytes
   size_t d_args_cap = 0;
-  char* h_args = nullptr;  // pinned staging of the descriptors
+  char* h_args = nullptr;    // pinned staging of the two sets
   size_t h_args_cap = 0;
   ox::TileMap* d_map = nullptr;
   size_t d_map_cap = 0;
-  // captured launches of one step for the current set of cases (the
-  // descriptors they read are refreshed before every replay)
-  hipGraphExec_t graph = nullptr;
+  int phase = 0;             // the descriptor set the next step reads
+  // captured launches of one step per descriptor set for the current cases
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
   // autotune cache: (precision, stage key, nx, ny, nz) -> (lseg_y, lseg_z)
   std::map<std::tuple<int, int, int, int, int>, std::pair<int, int>> tuned;
   // statistics
@@ -115,8 +118,10 @@ struct ox_batch {
 namespace {
 
 void drop_graphs(ox_batch* b) {
-  if (b->graph) hipGraphExecDestroy(b->graph);
-  b->graph = nullptr;
+  for (hipGraphExec_t& g : b->graph) {
+    if (g) hipGraphExecDestroy(g);
+    g = nullptr;
+  }
 }
 
 void invalidate(ox_batch* b) {
@@ -253,37 +258,93 @@ int build_plan(ox_batch* b) {
       off = (off + 255) & ~(size_t)255;
     }
   }
-  b->args_bytes = off;
-  int rc = ensure_device(&b->d_args, &b->d_args_cap, std::max<size_t>(off, 256));
+  b->args_bytes = std::max<size_t>(off, 256);
+  int rc = ensure_device(&b->d_args, &b->d_args_cap, 2 * b->args_bytes);
   if (rc != OX_OK) return rc;
-  if (off > b->h_args_cap) {
+  if (2 * b->args_bytes > b->h_args_cap) {
     if (b->h_args) hipHostFree(b->h_args);
     b->h_args = nullptr;
     b->h_args_cap = 0;
-    if (hipHostMalloc((void**)&b->h_args, off, hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void**)&b->h_args, 2 * b->args_bytes, hipHostMallocDefault) != hipSuccess)
       return ox_fail(OX_ENOMEM, "pinned staging allocation failed");
-    b->h_args_cap = off;
+    b->h_args_cap = 2 * b->args_bytes;
   }
   for (size_t li = 0; li < b->launches.size(); ++li) {
     const BGroup& G = b->groups[b->launch_grp[li]];
     const size_t n = G.members.size();
-    if (G.dim == 3) {
-      for (int d = 0; d < 3; ++d)
-        b->launches[li].args[d] = b->d_args + b->args_off[li] + (size_t)d * n * G.ops.args_bytes;
-    } else {
-      b->launches[li].args[0] = b->d_args + b->args_off[li];
+    if (G.dim == 2) {  // the same descriptors in both sets
       const size_t ab = ox2::args2_bytes(G.prec);
-      int tile0 = 0;
-      for (size_t m = 0; m < n; ++m) {
-        ox2d_solver* s2 = G.members[m]->s2;
-        ox2::write_args2(s2->cfg, s2->g, s2->buf, b->launch_stage[li], tile0, b->h_args + b->args_off[li] + m * ab);
-        tile0 += s2->ntiles;
+      for (int set = 0; set < 2; ++set) {
+        int tile0 = 0;
+        for (size_t m = 0; m < n; ++m) {
+          ox2d_solver* s2 = G.members[m]->s2;
+          ox2::write_args2(s2->cfg, s2->g, s2->buf, b->launch_stage[li], tile0,
+                           b->h_args + set * b->args_bytes + b->args_off[li] + m * ab);
+          tile0 += s2->ntiles;
+        }
       }
     }
-    b->launches[li].ep = 0;  // set per step
   }
+  // descriptors of the 3-D groups for the next two steps: each case's
+  // primitive-set parity advances stage by stage exactly as in ox_step, then
+  // returns to where it is now
+  std::vector<std::pair<int, const double*>> saved;
+  for (auto& up : b->active)
+    if (up->s) saved.push_back({up->s->C.cur, up->s->C.prim_for});
+  for (int set = 0; set < 2; ++set) {
+    for (size_t li = 0; li < b->launches.size(); ++li) {
+      BGroup& G = b->groups[b->launch_grp[li]];
+      if (G.dim == 2) continue;
+      const int st = b->launch_stage[li];
+      const size_t n = G.members.size(), ab = G.ops.args_bytes;
+      char* base = b->h_args + set * b->args_bytes + b->args_off[li];
+      for (size_t m = 0; m < n; ++m) {
+        ox_solver* s = G.members[m]->s;
+        const ox::Geom& g = s->C.g;
+        const double* Qsrc;
+        ox::StageArgs sa;
+        double coef;
+        rk_stage(s, st, &Qsrc, &sa, &coef);
+        if (s->C.prim_for != Qsrc) return ox_fail(OX_ESTATE, "batched case without current primitives");
+        alignas(16) char tmp[3 * kMaxArgs];
+        G.ops.make_args(s->C, sa, coef * g.a_yz, coef * g.a_zx, coef * g.a_xy, tmp);
+        for (int d = 0; d < 3; ++d) std::memcpy(base + ((size_t)d * n + m) * ab, tmp + (size_t)d * ab, ab);
+        b->launches[li].ep = sa.ep;
+        ox::finish_stage(s->C, sa);
+      }
+    }
+  }
+  {
+    size_t k = 0;
+    for (auto& up : b->active)
+      if (up->s) {
+        up->s->C.cur = saved[k].first;
+        up->s->C.prim_for = saved[k].second;
+        ++k;
+      }
+  }
+  OX_HIP(hipMemcpyAsync(b->d_args, b->h_args, 2 * b->args_bytes, hipMemcpyHostToDevice, b->stream));
+  OX_HIP(hipStreamSynchronize(b->stream));
+  b->phase = 0;
   b->plan_valid = true;
   return OX_OK;
+}
+
+// Advances every 3-D case's host-side stage bookkeeping (primitive-set
+// parity) by `steps` RK steps, matching what the device did.
+void advance_parity(ox_batch* b, int steps) {
+  for (auto& up : b->active) {
+    ox_solver* s = up->s;
+    if (!s) continue;
+    for (int k = 0; k < steps; ++k)
+      for (int st = 0; st < s->C.cfg.rk; ++st) {
+        const double* Qsrc;
+        ox::StageArgs sa;
+        double coef;
+        rk_stage(s, st, &Qsrc, &sa, &coef);
+        ox::finish_stage(s->C, sa);
+      }
+  }
 }
 
 // Segment lengths of a case from the engine's cache, tuning on a miss.
@@ -303,12 +364,19 @@ int tune_case(ox_batch* b, ox_solver* s) {
   return rc;
 }
 
-void enqueue_step(ox_batch* b) {
+// The launches of one step reading descriptor set `set`.
+void enqueue_step(ox_batch* b, int set) {
   for (size_t li = 0; li < b->launches.size(); ++li) {
     const BGroup& G = b->groups[b->launch_grp[li]];
-    const ox::BatchGroupLaunch& L = b->launches[li];
-    if (G.dim == 3) G.ops.launch(L);
-    else G.launch2[b->launch_stage[li]].launch(L.args[0], L.ncase, L.ntiles[0], L.grid[0], L.stream);
+    ox::BatchGroupLaunch L = b->launches[li];
+    char* base = b->d_args + set * b->args_bytes + b->args_off[li];
+    if (G.dim == 3) {
+      const size_t n = G.members.size();
+      for (int d = 0; d < 3; ++d) L.args[d] = base + (size_t)d * n * G.ops.args_bytes;
+      G.ops.launch(L);
+    } else {
+      G.launch2[b->launch_stage[li]].launch(base, L.ncase, L.ntiles[0], L.grid[0], L.stream);
+    }
   }
 }
 
@@ -503,9 +571,11 @@ int ox_batch_admit2d(ox_batch* b, const ox2d_config* cfg, const double* Q, int32
   return OX_OK;
 }
 
-int ox_batch_step(ox_batch* b, int32_t* active) {
+int ox_batch_run(ox_batch* b, int32_t max_steps, int32_t* steps_done, int32_t* active) {
   if (!b) return ox_fail(OX_EINVAL, "null handle");
+  if (max_steps < 1) return ox_fail(OX_EINVAL, "max_steps must be >= 1");
   OX_HIP(hipSetDevice(b->cfg.device));
+  if (steps_done) *steps_done = 0;
   if (b->active.empty()) {
     if (active) *active = 0;
     return OX_OK;
@@ -514,65 +584,53 @@ int ox_batch_step(ox_batch* b, int32_t* active) {
     int rc = build_plan(b);
     if (rc != OX_OK) return rc;
   }
-  // descriptors of every stage of the step; each case's primitive-set parity
-  // advances stage by stage exactly as in ox_step
-  for (size_t li = 0; li < b->launches.size(); ++li) {
-    BGroup& G = b->groups[b->launch_grp[li]];
-    if (G.dim == 2) continue;  // written with the plan
-    const int st = b->launch_stage[li];
-    const size_t n = G.members.size(), ab = G.ops.args_bytes;
-    char* base = b->h_args + b->args_off[li];
-    for (size_t m = 0; m < n; ++m) {
-      ox_solver* s = G.members[m]->s;
-      const ox::Geom& g = s->C.g;
-      const double* Qsrc;
-      ox::StageArgs sa;
-      double coef;
-      rk_stage(s, st, &Qsrc, &sa, &coef);
-      if (s->C.prim_for != Qsrc) return ox_fail(OX_ESTATE, "batched case without current primitives");
-      alignas(16) char tmp[3 * kMaxArgs];
-      G.ops.make_args(s->C, sa, coef * g.a_yz, coef * g.a_zx, coef * g.a_xy, tmp);
-      for (int d = 0; d < 3; ++d) std::memcpy(base + ((size_t)d * n + m) * ab, tmp + (size_t)d * ab, ab);
-      b->launches[li].ep = sa.ep;
-      ox::finish_stage(s->C, sa);
-    }
-  }
-  OX_HIP(hipMemcpyAsync(b->d_args, b->h_args, b->args_bytes, hipMemcpyHostToDevice, b->stream));
-  OX_HIP(hipEventRecord(b->ev0, b->stream));
-  bool replayed = false;
+  // steps until the first case finishes (the set of cases changes there)
+  int k = max_steps;
+  for (auto& c : b->active) k = std::min(k, c->remaining);
   if (b->cfg.use_graph) {
-    if (!b->graph) {
-      // capture the step's launches once for this set of cases
+    for (int set = 0; set < 2; ++set) {
+      if (b->graph[set]) continue;
+      // capture the step's launches once per descriptor set for this set of cases
       hipGraph_t graph = nullptr;
       OX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      enqueue_step(b);
+      enqueue_step(b, set);
       OX_HIP(hipStreamEndCapture(b->stream, &graph));
-      const hipError_t e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+      const hipError_t e = hipGraphInstantiate(&b->graph[set], graph, nullptr, nullptr, 0);
       hipGraphDestroy(graph);
       if (e != hipSuccess) {
-        b->graph = nullptr;
+        b->graph[set] = nullptr;
         return ox_fail(OX_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
       }
       ++b->graph_captures;
     }
-    OX_HIP(hipGraphLaunch(b->graph, b->stream));
-    ++b->graph_replays;
-    replayed = true;
   }
-  if (!replayed) enqueue_step(b);
+  OX_HIP(hipEventRecord(b->ev0, b->stream));
+  for (int t = 0; t < k; ++t) {
+    const int set = (b->phase + t) & 1;
+    if (b->cfg.use_graph) {
+      OX_HIP(hipGraphLaunch(b->graph[set], b->stream));
+      ++b->graph_replays;
+    } else {
+      enqueue_step(b, set);
+    }
+  }
   OX_HIP(hipGetLastError());
   OX_HIP(hipEventRecord(b->ev1, b->stream));
   OX_HIP(hipEventSynchronize(b->ev1));
   float ms = 0.f;
   OX_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->device_ms += ms;
-  ++b->steps;
+  b->steps += k;
+  b->phase = (b->phase + k) & 1;
+  advance_parity(b, k);
+  if (steps_done) *steps_done = k;
   // bookkeeping: stage counts, finished cases
   bool changed = false;
   for (size_t a = 0; a < b->active.size();) {
     BCase* c = b->active[a].get();
-    b->cell_stages += c->cells() * c->stages();
-    if (--c->remaining == 0) {
+    b->cell_stages += c->cells() * c->stages() * k;
+    c->remaining -= k;
+    if (c->remaining == 0) {
       // the result is read with its ghost layers (2-D: written by the last stage)
       if (c->s) ox::launch_ghost_fill(c->s->C, c->s->QJ);
       b->done.push_back(std::move(b->active[a]));
@@ -589,6 +647,8 @@ int ox_batch_step(ox_batch* b, int32_t* active) {
   if (active) *active = (int32_t)b->active.size();
   return OX_OK;
 }
+
+int ox_batch_step(ox_batch* b, int32_t* active) { return ox_batch_run(b, 1, nullptr, active); }
 
 int ox_batch_done(ox_batch* b, int64_t* ids, int32_t cap, int32_t* n) {
   if (!b || !n) return ox_fail(OX_EINVAL, "null argument");

@@ -67,6 +67,32 @@ def test_alone_equals_in_mix(precision, use_graph):
         assert st["graph_replays"] == step and st["graph_captures"] >= 1
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_bulk"])
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_run_back_to_back_equals_alone(precision, use_graph):
+    """ox_batch_run: many steps enqueued back to back (alternating between the
+    two descriptor sets of RK3 cases, one set for RK4) end bit-identical to
+    each case run alone; runs stop at every step where a case finishes."""
+    mix = [(s, n) for s, n, _ in _mix_specs(precision)]
+    states = [cases.initial_state(s) for s, _ in mix]
+    ref = [_alone(s, Q, n) for (s, n), Q in zip(mix, states)]
+    out = {}
+    runs = []
+    with Batch(hbm_budget=8 * GIB, max_cases=16, use_graph=use_graph, autotune=False) as b:
+        ids = {b.admit(s, Q, n): k for k, ((s, n), Q) in enumerate(zip(mix, states))}
+        while len(out) < len(mix):
+            k, act = b.run(100)
+            runs.append(k)
+            for cid in b.done():
+                out[ids[cid]] = b.retire(cid)
+        st = b.stats()
+    for k in range(len(mix)):
+        np.testing.assert_array_equal(out[k], ref[k], err_msg=f"case {k}")
+    ends = sorted(set(n for _, n in mix))
+    assert runs == [e - p for p, e in zip([0] + ends[:-1], ends)]
+    assert st["steps"] == max(ends)
+
+
 def test_batched_fp64_equals_oracle():
     spec = cases.tgv_spec(20, 6, Ma=1.25, cfl=0.3, cfl_mode="acoustic", scheme="SLAU", visc="NS", visc_order=6, rk=3,
                           tvd="hybrid")

@@ -9,6 +9,10 @@ cmake/make to compile one executable, it reads the patched files into an
 ``run()`` drives the library in-process through ctypes instead of
 ``mpirun a.out`` and writes the same run directory: ``data/Q%05d.vtr``,
 ``data/kinetic_energy.d``, ``data/entropy.d`` and ``recal/Q00001.dat``.
+Two-dimensional cases (``dimension = 2`` in mod_globals.f90, the
+2D_solver/OS oblique-shock case) run on the 2-D path and write what the
+reference's 2-D executable writes: ``data/Q%05d.vtr`` (one z plane) and
+``recal/Q00001.dat``, ``recal/x.dat``, ``recal/y.dat``.
 """
 from __future__ import annotations
 
@@ -17,8 +21,11 @@ import pathlib
 import shutil
 from typing import Any
 
+import numpy as np
+
 from . import io_vtr
 from .cases import CaseSpec, cell_centres
+from .cases2d import CaseSpec2D, os_state
 from .fortran_params import read_fypp_macros, read_parameters
 from .ic import bc_to_params, ic_to_params, initial_state
 from .patcher import _is_macro_line, patch
@@ -58,7 +65,7 @@ class Case:
             raw.update(bc_to_params(bc_spec))
         self.params = {_ALIAS.get(k.lower(), k): v for k, v in raw.items()}
         self._built = False
-        self.spec: CaseSpec | None = None
+        self.spec: CaseSpec | CaseSpec2D | None = None
         self.globals: dict = {}
         self.macros: dict = {}
 
@@ -108,8 +115,10 @@ class Case:
              "spec": {k: getattr(self.spec, k) for k in self.spec.__dataclass_fields__}}, indent=1, default=str))
         self._built = True
 
-    def _make_spec(self) -> CaseSpec:
+    def _make_spec(self):
         m, g = self.macros, self.globals
+        if int(g.get("dimension", 3)) == 2:
+            return self._make_spec2d()
         for flag in ("BC_X", "BC_Y", "BC_Z"):
             if m.get(flag, False):
                 raise RuntimeError(f"{flag}=True: only periodic boundaries are built into the gfx950 library")
@@ -130,6 +139,70 @@ class Case:
             spec.visc_order = 0
         return spec
 
+    def _make_spec2d(self) -> CaseSpec2D:
+        """The 2-D oblique-shock case: non-periodic x and y (BC_X, BC_Y) with
+        the boundary conditions of 2D_solver/OS/set.f90:48-86."""
+        m, g = self.macros, self.globals
+        if self.source.name != "OS":
+            raise RuntimeError(f"2-D case {self.source.name!r}: only the oblique-shock case (OS) is built")
+        if not (m.get("BC_X", True) and m.get("BC_Y", True)):
+            raise RuntimeError("the 2-D path runs the oblique-shock boundaries (BC_X = BC_Y = True)")
+        if str(m.get("VISC", "Euler")) != "Euler":
+            raise RuntimeError("the 2-D path is inviscid (VISC = 'Euler')")
+        if int(m.get("RK", 3)) != 3:
+            raise RuntimeError("the 2-D path advances with RK3")
+        return CaseSpec2D(
+            nx=int(g["nx"]), ny=int(g["ny"]), order=int(m.get("ORDER", 6)), scheme=str(m.get("SCHEME", "SLAU")),
+            tvd=str(m.get("TVD", "hybrid")), slau_variant=str(m.get("SLAU_VARIANT", "HRSLAU2")), rk=3,
+            precision=str(m.get("PRECISION", "fp64")), gamma=float(g.get("gamma", 1.4)),
+            R=float(g.get("r", 287.03)), threshold=float(g.get("threshold", 0.4)), dt=float(g["dt"]),
+            Lx=float(g["lx"]), Ly=float(g["ly"]), M=float(g.get("m0", 2.0)), p_tot=float(g.get("p_tot", 100e3)),
+            T_tot=float(g.get("t_tot", 295.0)), beta=float(g["beta"]))
+
+    def _run2d(self) -> None:
+        """RungeKutta of the 2-D executable (2D_solver/src/calc_time_dev.f90.fypp:74-102):
+        np outputs of nt steps, one .vtr per output (print_vtk_2D,
+        src/print.f90:351-361), then the restart and grid files
+        (2D_solver/src/main.f90:80-99)."""
+        from .solver2d import Solver2D
+        spec, g = self.spec, self.globals
+        data = self.workdir / "data"
+        data.mkdir(exist_ok=True)
+        recal = self.workdir / "recal"
+        recal.mkdir(exist_ok=True)
+        nt, np_ = int(g.get("nt", 1)), int(g.get("np", 1))
+        offset = int(g.get("step_offset", 0))
+        if bool(self.macros.get("RESTART", False)):
+            Q = io_vtr.read_restart(recal / "Q00001.dat", spec.state_shape())
+        else:
+            Q = os_state(spec)
+        # set_grid (OS/set.f90:8-26): running sums of the uniform spacing
+        x = np.zeros(spec.nx)
+        y = np.zeros(spec.ny)
+        dx0, dy0 = spec.Lx / float(spec.nx - 1), spec.Ly / float(spec.ny - 1)
+        for i in range(1, spec.nx):
+            x[i] = x[i - 1] + dx0
+        for j in range(1, spec.ny):
+            y[j] = y[j - 1] + dy0
+        z = np.zeros(1)
+        with Solver2D(spec) as s:
+            s.set_state(Q)
+
+            def output(step):
+                rho, p, vel = io_vtr.primitives_for_print2d(s.get_state(), spec.gamma)
+                io_vtr.write_vtr(data / f"Q{step + offset:05d}.vtr", x, y, z, rho, p, vel)
+
+            output(0)
+            for t2 in range(1, np_ + 1):
+                s.step(nt)
+                output(t2)
+            Qf = s.get_state()
+        if not np.isfinite(Qf).all():
+            raise RuntimeError("non-finite state at the end of the 2-D run")
+        io_vtr.write_restart(recal / "Q00001.dat", Qf)
+        (recal / "x.dat").write_bytes(np.ascontiguousarray(x, dtype="<f8").tobytes())
+        (recal / "y.dat").write_bytes(np.ascontiguousarray(y, dtype="<f8").tobytes())
+
     # -- run -------------------------------------------------------------------
     def run(self, nranks: int = 2) -> None:
         """Advance np outputs x nt steps on one GPU, writing the run directory.
@@ -137,6 +210,9 @@ class Case:
         replaced by in-process writes)."""
         if not self._built:
             raise RuntimeError("Must call build() before run()")
+        if isinstance(self.spec, CaseSpec2D):
+            self._run2d()
+            return
         from .solver import Solver
         spec = self.spec
         g = self.globals

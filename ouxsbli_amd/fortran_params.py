@@ -9,7 +9,8 @@ import re
 
 _FUNCS = {
     "sqrt": math.sqrt, "dsqrt": math.sqrt, "sin": math.sin, "dsin": math.sin, "cos": math.cos, "dcos": math.cos,
-    "tan": math.tan, "dtan": math.tan, "atan": math.atan, "datan": math.atan, "acos": math.acos, "asin": math.asin,
+    "tan": math.tan, "dtan": math.tan, "atan": math.atan, "datan": math.atan, "acos": math.acos, "dacos": math.acos,
+    "asin": math.asin, "dasin": math.asin, "dexp": math.exp, "dlog": math.log,
     "exp": math.exp, "log": math.log, "abs": abs, "dabs": abs, "dble": float, "real": lambda x, *a, **k: float(x),
     "int": int, "nint": round, "min": min, "max": max, "mod": math.fmod, "kind": lambda x: 8,
 }

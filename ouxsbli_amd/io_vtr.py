@@ -54,6 +54,17 @@ def primitives_for_print(QJ: np.ndarray, jac: float, gamma: float):
     return rho.astype(np.float32), p.astype(np.float32), vel.astype(np.float32)
 
 
+def primitives_for_print2d(Q: np.ndarray, gamma: float):
+    """make_1d_for_print2 (src/print.f90:192-215) from the physical state
+    Q(nx,4,ny) (numpy shape (ny, 4, nx)): FP64 recovery, FP32 fields, w = 0."""
+    rho = Q[:, 0, :]
+    u = Q[:, 1, :] / rho
+    v = Q[:, 2, :] / rho
+    p = (gamma - 1.0) * (Q[:, 3, :] - 0.5 * rho * (u ** 2 + v ** 2))
+    vel = np.stack([u, v, np.zeros_like(u)], axis=-1)[None]  # (1, ny, nx, 3)
+    return rho[None].astype(np.float32), p[None].astype(np.float32), vel.astype(np.float32)
+
+
 def write_vtr(path, x, y, z, rho, p, vel, dimension: int = 3) -> None:
     """print_xml (src/print.f90:125-189) with io = 4 (Float32)."""
     ni, nj, nk = len(x), len(y), len(z)

@@ -49,3 +49,38 @@ def test_restart_continues_bitwise(tmp_path):
     again = io_vtr.read_restart(tmp_path / "c" / "recal" / "Q00001.dat", c.spec.state_shape())
     # the restart file holds J*QJ, re-divided by J on load: equal to ~1 ulp
     np.testing.assert_allclose(again, full, rtol=1e-13, atol=0)
+
+
+def test_oblique_shock_case_like_the_reference_test(tmp_path):
+    """The reference's OS test (ouxsbli/tests/test_os.py) through the Case API:
+    same parameters, one output after 0.1 ms, region means of the written
+    .vtr within 3 % of the analytic free stream and reflected-shock states."""
+    import math
+    from ouxsbli_amd import cases2d as c2
+    from test_oracle2d import reference_reflected_shock
+    nx, ny, endT, dt = 257, 129, 0.1e-3, 3.0e-9
+    case = Case("2D_solver/OS", str(tmp_path / "os"), scheme="slau", accuracy=6, visc="euler", tvd="hybrid",
+                nx=nx, ny=ny, Lx=5e-3, Ly=2e-3, gamma=1.4, Pr=0.72, R=287.03, M0=2.0, p_tot=100e3, T_tot=295.0,
+                beta=math.pi * 37.2 / 180.0, dt=dt, np=1, nt=int(endT / dt))
+    case.build()
+    case.run(nranks=2)
+    d = tmp_path / "os" / "data"
+    assert (d / "Q00000.vtr").exists()
+    x, y, z, rho, p, vel = io_vtr.read_vtr(d / "Q00001.vtr")
+    ni, nj, nk = len(x), len(y), len(z)
+    assert (ni, nj, nk) == (nx, ny, 1)
+    u, v = vel[..., 0], vel[..., 1]
+    u0, p0, T0, rho0 = c2.free_stream()
+    pre = [a[0, nj // 2:, : ni // 10].astype(float).mean() for a in (rho, u, p)]
+    for got, want in zip(pre, (rho0, u0, p0)):
+        assert abs(got - want) / want < 0.03
+    rho3, ux3, p3 = reference_reflected_shock(2.0, 1.4, 287.03, 100e3, 295.0, math.pi * 37.2 / 180.0,
+                                              math.pi * 44.1 / 180.0)
+    post = [a[0, : nj // 3, 9 * ni // 10:].astype(float).mean() for a in (rho, u, v, p)]
+    assert abs(post[0] - rho3) / rho3 < 0.03
+    assert abs(post[1] - ux3) / ux3 < 0.03
+    assert abs(post[2]) / abs(ux3) < 0.03
+    assert abs(post[3] - p3) / p3 < 0.03
+    recal = tmp_path / "os" / "recal"
+    assert (recal / "Q00001.dat").stat().st_size == nx * 4 * ny * 8
+    assert (recal / "x.dat").stat().st_size == nx * 8

@@ -4,9 +4,13 @@ Requests arrive at their own times (a seeded Poisson process in the
 benchmark).  Arrived requests queue first-in first-out; the head of the
 queue is admitted as soon as the engine has a free slot and enough free HBM
 budget for its footprint (``Batch.admit`` refuses otherwise), later requests
-wait behind it.  Every loop iteration advances all resident cases by one RK
-step together (continuous batching: a case joins at the next step after its
-admission and leaves the step after its last), then retires finished cases.
+wait behind it.  Every loop iteration advances all resident cases together
+(continuous batching: a case joins at the next step after its admission and
+leaves the step after its last), then retires finished cases.  An iteration
+runs as many steps back to back as fit before the next arrival, at the
+measured step time, and stops early when a case finishes, so arrivals are
+admitted within about one step and the device is not idled by per-step
+host round trips.
 Latency of a request = completion time - arrival time.  The reference has
 no serving path (one case per ``mpirun``, ouxsbli/case.py:180-201).
 """
@@ -99,7 +103,7 @@ class ServeReport:
 def serve(requests: List[Request], hbm_budget: int, max_cases: int = 64, use_graph: bool = True,
           autotune: bool = True, pretune: bool = True, keep_results: bool = True,
           clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] = time.sleep,
-          batch: Optional[Batch] = None) -> ServeReport:
+          batch: Optional[Batch] = None, max_run: int = 1024) -> ServeReport:
     """Runs the stream to completion and returns the per-request timings.
 
     ``pretune`` fills the engine's segment-length cache for every distinct
@@ -125,6 +129,7 @@ def serve(requests: List[Request], hbm_budget: int, max_cases: int = 64, use_gra
         by_id: Dict[int, Request] = {}
         active = 0
         steps = 0
+        step_s = None  # running estimate of the wall time of one batch step
         t0 = clock()
         while pending or waiting or active:
             now = clock() - t0
@@ -144,8 +149,18 @@ def serve(requests: List[Request], hbm_budget: int, max_cases: int = 64, use_gra
                 if pending:
                     sleep(max(0.0, pending[0].arrival - (clock() - t0)))
                 continue
-            active = batch.step()
-            steps += 1
+            if pending and step_s is not None:
+                k = int((pending[0].arrival - (clock() - t0)) / step_s)
+                k = max(1, min(max_run, k))
+            elif pending:
+                k = 1
+            else:
+                k = max_run  # stops at the first finish, which is when a waiting request can enter
+            t_run = clock()
+            ran, active = batch.run(k)
+            per = (clock() - t_run) / max(ran, 1)
+            step_s = per if step_s is None else 0.7 * step_s + 0.3 * per
+            steps += ran
             for cid in batch.done():
                 r = by_id.pop(cid)
                 Q = batch.retire(cid)

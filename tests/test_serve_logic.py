@@ -51,6 +51,7 @@ class FakeBatch:
 
     def step(self):
         self.clock.t += self.step_s
+        self.log.append(("step", None, self.clock()))
         for cid in list(self.active):
             self.active[cid][0] -= 1
             if self.active[cid][0] == 0:
@@ -58,6 +59,17 @@ class FakeBatch:
         for cid in self.finished:
             self.active.pop(cid, None)
         return len(self.active)
+
+    def run(self, k):
+        """Up to k steps, stopping after the first one in which a case finishes."""
+        n = 0
+        while n < k and self.active:
+            before = len(self.finished)
+            self.step()
+            n += 1
+            if len(self.finished) > before:
+                break
+        return n, len(self.active)
 
     def done(self):
         return list(self.finished)
@@ -130,3 +142,18 @@ def test_pretune_visits_each_shape_once():
     reqs = _reqs([0.0, 0.1, 0.2], [8, 8, 16], [1, 1, 1])
     serve.serve(reqs, hbm_budget=0, batch=fb, clock=clock, sleep=clock.sleep, autotune=True, pretune=True)
     assert len(fb.tuned) == 2
+
+
+def test_runs_back_to_back_between_arrivals():
+    """With nothing arriving for a while the loop runs many steps per
+    iteration (one host round trip), and a request arriving mid-run is still
+    admitted within a couple of steps of its arrival."""
+    clock = FakeClock()
+    fb = FakeBatch(clock, budget=10 ** 9, slots=4, step_s=0.01)
+    reqs = _reqs([0.0, 0.505], [8, 8], [100, 3])
+    rep = serve.serve(reqs, hbm_budget=0, batch=fb, clock=clock, sleep=clock.sleep, pretune=False)
+    r0, r1 = rep.requests
+    assert r0.finished == pytest.approx(1.0)
+    assert 0.505 <= r1.admitted <= 0.505 + 2 * 0.01 + 1e-9
+    assert r1.finished == pytest.approx(r1.admitted + 0.03)
+    assert rep.steps == 100

@@ -26,6 +26,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 
 def _opcounts():
@@ -121,6 +122,17 @@ def les_leg(n, steps):
     out["fp32_bulk"]["ke_rel_err_vs_fp64"] = abs(k32 - k64) / abs(k64)
     out["fp32_bulk"]["speedup_vs_fp64"] = round(out["fp32_bulk"]["Mcell_updates_per_s_stage"] /
                                                 out["fp64"]["Mcell_updates_per_s_stage"], 3)
+    # the fused stage kernel against both roofs: compulsory FP64 state bytes and
+    # the algorithmic operation count of smooth faces (ouxsbli_amd/flops/opcount.json;
+    # limited faces cost more, so this understates the work done)
+    ops = _opcounts()["roe6_hybrid_2d_rk3"]["per_cell_stage_smooth"]
+    rate = out["fp32_bulk"]["device_busy_Mcell_updates_per_s_stage"] * 1e6
+    out["fp32_bulk"]["roofline"] = {
+        "kernel": "k2d_stage<float,6,ROE,*>", "bound": "hbm", "unit": "GB/s",
+        "achieved": round(rate * OS_BYTES_PER_CELL_STAGE / 1e9, 1), "peak": HBM_PEAK_GBS,
+        "frac": round(rate * OS_BYTES_PER_CELL_STAGE / 1e9 / HBM_PEAK_GBS, 4),
+        "ops_per_cell_stage": ops, "achieved_TFLOPs": round(rate * ops / 1e12, 2), "peak_TFLOPs": FP32_PEAK_TFLOPS,
+        "flop_frac": round(rate * ops / 1e12 / FP32_PEAK_TFLOPS, 4)}
     out["state_ok"] = bool(res["fp32_bulk"]["ok"] and res["fp64"]["ok"])
     return out
 
@@ -184,6 +196,17 @@ def batched_os_leg(ncase, steps, unbatched_steps, cpu_seconds):
         out["cell_stages"] = cs
     out["fp32_bulk"]["speedup_vs_fp64"] = round(out["fp32_bulk"]["Mcell_updates_per_s_stage"] /
                                                 out["fp64"]["Mcell_updates_per_s_stage"], 3)
+    # the fused stage kernel against both roofs: compulsory FP64 state bytes and
+    # the algorithmic operation count of smooth faces (ouxsbli_amd/flops/opcount.json;
+    # limited faces cost more, so this understates the work done)
+    ops = _opcounts()["roe6_hybrid_2d_rk3"]["per_cell_stage_smooth"]
+    rate = out["fp32_bulk"]["device_busy_Mcell_updates_per_s_stage"] * 1e6
+    out["fp32_bulk"]["roofline"] = {
+        "kernel": "k2d_stage<float,6,ROE,*>", "bound": "hbm", "unit": "GB/s",
+        "achieved": round(rate * OS_BYTES_PER_CELL_STAGE / 1e9, 1), "peak": HBM_PEAK_GBS,
+        "frac": round(rate * OS_BYTES_PER_CELL_STAGE / 1e9 / HBM_PEAK_GBS, 4),
+        "ops_per_cell_stage": ops, "achieved_TFLOPs": round(rate * ops / 1e12, 2), "peak_TFLOPs": FP32_PEAK_TFLOPS,
+        "flop_frac": round(rate * ops / 1e12 / FP32_PEAK_TFLOPS, 4)}
     # accuracy of the FP32-bulk ensemble against the FP64 one, per case
     errs = []
     for a, b in zip(res["fp64"], res["fp32_bulk"]):

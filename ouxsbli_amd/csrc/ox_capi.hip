@@ -70,6 +70,14 @@ __global__ __launch_bounds__(256) void k_scale(double* __restrict__ Q, long long
   Q[t] = divide ? Q[t] / J : J * Q[t];
 }
 
+// dst = J * src (the physical state of ox_get_state, written to scratch).
+__global__ __launch_bounds__(256) void k_scale_to(const double* __restrict__ src, double* __restrict__ dst, long long n,
+                                                  double J) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  dst[t] = J * src[t];
+}
+
 }  // namespace ox
 
 namespace {
@@ -343,12 +351,14 @@ int ox_get_state(ox_solver* s, double* Q) {
   if (!s || !Q) return ox_fail(OX_EINVAL, "null argument");
   OX_HIP(hipSetDevice(s->C.cfg.device));
   const long long n = (long long)state_elems(s->C.g);
-  // J * QJ into the scratch-free second state buffer is not allowed (RK state);
-  // scale on the host instead.
-  OX_HIP(hipMemcpyAsync(Q, s->QJ, n * sizeof(double), hipMemcpyDeviceToHost, s->C.stream));
+  // J * QJ into the second state buffer, which holds no live data between
+  // calls (every RK step rewrites it from QJ before reading it), then one copy
   OX_HIP(hipStreamSynchronize(s->C.stream));
-  const double J = s->C.g.jac;
-  for (long long t = 0; t < n; ++t) Q[t] = J * Q[t];
+  ox::k_scale_to<<<(unsigned)((n + 255) / 256), 256, 0, s->C.stream>>>(s->QJ, s->QJ2, n, s->C.g.jac);
+  OX_HIP(hipGetLastError());
+  if (s->C.prim_for == s->QJ2) s->C.prim_for = nullptr;
+  OX_HIP(hipMemcpyAsync(Q, s->QJ2, n * sizeof(double), hipMemcpyDeviceToHost, s->C.stream));
+  OX_HIP(hipStreamSynchronize(s->C.stream));
   return OX_OK;
 }
 

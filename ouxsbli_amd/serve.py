@@ -77,6 +77,7 @@ class ServeReport:
     wall_s: float
     batch_stats: Dict
     steps: int
+    host_s: Dict = dataclasses.field(default_factory=dict)  # wall seconds in admit / run / retire calls
 
     def latencies(self) -> np.ndarray:
         return np.array([r.finished - r.arrival for r in self.requests])
@@ -97,6 +98,7 @@ class ServeReport:
             "batch_steps": self.steps,
             "graph_captures": self.batch_stats["graph_captures"],
             "graph_replays": self.batch_stats["graph_replays"],
+            **{f"{k}_s": round(v, 4) for k, v in self.host_s.items()},
         }
 
 
@@ -130,6 +132,7 @@ def serve(requests: List[Request], hbm_budget: int, max_cases: int = 64, use_gra
         active = 0
         steps = 0
         step_s = None  # running estimate of the wall time of one batch step
+        host = {"admit": 0.0, "run": 0.0, "retire": 0.0}
         t0 = clock()
         while pending or waiting or active:
             now = clock() - t0
@@ -137,10 +140,13 @@ def serve(requests: List[Request], hbm_budget: int, max_cases: int = 64, use_gra
                 waiting.append(pending.popleft())
             while waiting:
                 r = waiting[0]
+                ta = clock()
                 try:
                     cid = batch.admit(r.spec, r.Q0, r.nsteps)
                 except Busy:
+                    host["admit"] += clock() - ta
                     break
+                host["admit"] += clock() - ta
                 r.admitted = clock() - t0
                 by_id[cid] = r
                 waiting.popleft()
@@ -158,17 +164,20 @@ def serve(requests: List[Request], hbm_budget: int, max_cases: int = 64, use_gra
                 k = max_run  # stops at the first finish, which is when a waiting request can enter
             t_run = clock()
             ran, active = batch.run(k)
+            host["run"] += clock() - t_run
             per = (clock() - t_run) / max(ran, 1)
             step_s = per if step_s is None else 0.7 * step_s + 0.3 * per
             steps += ran
             for cid in batch.done():
                 r = by_id.pop(cid)
+                tr = clock()
                 Q = batch.retire(cid)
+                host["retire"] += clock() - tr
                 r.finished = clock() - t0
                 if keep_results:
                     r.result = Q
         wall = clock() - t0
-        return ServeReport(requests, wall, batch.stats(), steps)
+        return ServeReport(requests, wall, batch.stats(), steps, host)
     finally:
         if own:
             batch.close()

@@ -422,6 +422,29 @@ struct TileMap {
   int pad;
 };
 
+// Workgroups are dispatched round-robin over the eight XCDs, and each XCD has
+// its own L2.  For n workgroups with n a multiple of 8, tile
+// (b % 8) * (n / 8) + b / 8 gives XCD b % 8 one contiguous run of n / 8
+// tiles, so the tiles whose stencils overlap (neighbours along the second and
+// third launch dimensions) share an L2 instead of fetching the same rows into
+// two.  The map is a permutation of [0, n).
+#ifndef OX_XCD_SWIZZLE
+#define OX_XCD_SWIZZLE 1
+#endif
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  if (!OX_XCD_SWIZZLE || (n & 7)) return b;
+  return (b & 7) * (n >> 3) + (b >> 3);
+}
+
+// This workgroup's tile (bx, by, bz) of a 3-D launch under xcd_tile.
+__device__ __forceinline__ void launch_tile(int& bx, int& by, int& bz) {
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
+  bx = t % gx;
+  by = (t / gx) % gy;
+  bz = t / (gx * gy);
+}
+
 // Case owning global tile t (map[].start ascending, map[0].start == 0).
 __device__ __forceinline__ int find_case(const TileMap* __restrict__ map, int ncase, int t) {
   int lo = 0, hi = ncase - 1;
@@ -637,7 +660,9 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<R, S>& a, int bx, i
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
 __global__ __launch_bounds__(256) void k_sweep_x(SweepArgs<R, S> a) {
   __shared__ R tile[xsweep_nf<VISC>()][XROWS][64];
-  sweep_x_tile<R, S, ORDER, SCHEME, VISC, VO>(a, blockIdx.x, blockIdx.y, blockIdx.z, tile);
+  int bx, by, bz;
+  launch_tile(bx, by, bz);
+  sweep_x_tile<R, S, ORDER, SCHEME, VISC, VO>(a, bx, by, bz, tile);
 }
 
 // Persistent batched x sweep: block b handles global tiles b, b + gridDim.x, ...
@@ -645,7 +670,7 @@ template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
 __global__ __launch_bounds__(256) void k_sweep_x_batch(const SweepArgs<R, S>* __restrict__ cases,
                                                        const TileMap* __restrict__ map, int ncase, int ntiles) {
   __shared__ R tile[xsweep_nf<VISC>()][XROWS][64];
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (int t = xcd_tile(blockIdx.x, gridDim.x); t < ntiles; t += gridDim.x) {
     const int ci = find_case(map, ncase, t);
     const int local = t - map[ci].start, nbx = map[ci].nbx, nby = map[ci].nby;
     sweep_x_tile<R, S, ORDER, SCHEME, VISC, VO>(cases[ci], local % nbx, (local / nbx) % nby, local / (nbx * nby),
@@ -819,14 +844,16 @@ __device__ __forceinline__ void sweep_line_tile(const SweepArgs<R, S>& a, int bx
 
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO, int DIR, int EP>
 __global__ __launch_bounds__(256) void k_sweep_line(SweepArgs<R, S> a) {
-  sweep_line_tile<R, S, ORDER, SCHEME, VISC, VO, DIR, EP>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+  int bx, by, bz;
+  launch_tile(bx, by, bz);
+  sweep_line_tile<R, S, ORDER, SCHEME, VISC, VO, DIR, EP>(a, bx, by, bz);
 }
 
 // Persistent batched y / z sweep over the tiles of several cases.
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO, int DIR, int EP>
 __global__ __launch_bounds__(256) void k_sweep_line_batch(const SweepArgs<R, S>* __restrict__ cases,
                                                           const TileMap* __restrict__ map, int ncase, int ntiles) {
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  for (int t = xcd_tile(blockIdx.x, gridDim.x); t < ntiles; t += gridDim.x) {
     const int ci = find_case(map, ncase, t);
     const int local = t - map[ci].start, nbx = map[ci].nbx, nby = map[ci].nby;
     sweep_line_tile<R, S, ORDER, SCHEME, VISC, VO, DIR, EP>(cases[ci], local % nbx, (local / nbx) % nby,

@@ -342,10 +342,10 @@ __device__ __forceinline__ void load_cell(const Prim<S>& P, const Geom& g, const
   finish_cell<R, S, DIR, ORDER, SCHEME, VISC, VO, LOAD_S>(g, c, r, d);
 }
 
-// Face flux (convective minus viscous) from SW window cells w[0..SW-1]; the
-// face lies between w[HW-1] and w[HW].
+// Convective face flux from SW window cells w[0..SW-1]; the face lies
+// between w[HW-1] and w[HW].
 template <class R, int DIR, int ORDER, int SCHEME, int VISC, int VO>
-__device__ __forceinline__ void face_flux(const PhysC<R>& c, int tvd, int variant, R inv_d,
+__device__ __forceinline__ void face_conv(const PhysC<R>& c, int tvd, int variant,
                                           const CellData<R, DIR, ORDER, SCHEME, VISC, VO>* w, R* F) {
   using St = Sten<ORDER, VISC, VO>;
   constexpr int N = ORDER;
@@ -377,6 +377,13 @@ __device__ __forceinline__ void face_flux(const PhysC<R>& c, int tvd, int varian
       slau<R, DIR>(c, variant, L[0], Rr[0], L[1], Rr[1], L[2], Rr[2], L[3], Rr[3], L[4], Rr[4], R(1), F);
     }
   }
+}
+
+// Subtracts the viscous face flux from F (convective in, total out).
+template <class R, int DIR, int ORDER, int SCHEME, int VISC, int VO>
+__device__ __forceinline__ void face_visc(const PhysC<R>& c, R inv_d,
+                                          const CellData<R, DIR, ORDER, SCHEME, VISC, VO>* w, R* F) {
+  using St = Sten<ORDER, VISC, VO>;
   if constexpr (VISC) {
     constexpr int M = vwin<VO>();
     R mu_[M], T_[M], u_[M], v_[M], w_[M], a1[M], a2[M], a3[M], a4[M], mt_[M], H_[M];
@@ -395,6 +402,14 @@ __device__ __forceinline__ void face_flux(const PhysC<R>& c, int tvd, int varian
     F[3] = F[3] - tau[2];
     F[4] = F[4] - en;
   }
+}
+
+// Face flux (convective minus viscous) from SW window cells w[0..SW-1].
+template <class R, int DIR, int ORDER, int SCHEME, int VISC, int VO>
+__device__ __forceinline__ void face_flux(const PhysC<R>& c, int tvd, int variant, R inv_d,
+                                          const CellData<R, DIR, ORDER, SCHEME, VISC, VO>* w, R* F) {
+  face_conv<R, DIR, ORDER, SCHEME, VISC, VO>(c, tvd, variant, w, F);
+  face_visc<R, DIR, ORDER, SCHEME, VISC, VO>(c, inv_d, w, F);
 }
 
 // ---------------------------------------------------------------------------
@@ -463,6 +478,9 @@ __device__ __forceinline__ int find_case(const TileMap* __restrict__ map, int nc
 // need; the right-face flux of each cell comes from the neighbouring lane by
 // a cross-lane shuffle.
 // ---------------------------------------------------------------------------
+#ifndef OX_SPLIT_FENCE
+#define OX_SPLIT_FENCE() asm volatile("" ::: "memory")
+#endif
 template <int ORDER, int VISC, int VO> constexpr int xtile() { return 64 - Sten<ORDER, VISC, VO>::SW; }
 constexpr int XROWS = 4;
 
@@ -629,23 +647,36 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<R, S>& a, int bx, i
   const int f = i0 - 1 + tx;
   const bool face_ok = tx <= XT && f <= g.h + g.ix - 1;
   if (face_ok) {
-    CD win[St::SW];
+    // The convective and the viscous halves each read their window from the
+    // LDS row just before use, which keeps the two halves' inputs out of
+    // registers at the same time.
+    {
+      CD win[St::SW];
 #pragma unroll
-    for (int n = 0; n < St::SW; ++n) {
-      win[n].rho = tile[0][ty][tx + n]; win[n].u = tile[1][ty][tx + n]; win[n].v = tile[2][ty][tx + n];
-      win[n].w = tile[3][ty][tx + n]; win[n].p = tile[4][ty][tx + n];
-      if constexpr (kUsesT<SCHEME> || VISC) win[n].T = tile[5][ty][tx + n];
-      if constexpr (SENS) {
-        if (n == St::HW - 1 || n == St::HW) win[n].s = tile[7][ty][tx + n];
+      for (int n = 0; n < St::SW; ++n) {
+        win[n].rho = tile[0][ty][tx + n]; win[n].u = tile[1][ty][tx + n]; win[n].v = tile[2][ty][tx + n];
+        win[n].w = tile[3][ty][tx + n]; win[n].p = tile[4][ty][tx + n];
+        if constexpr (kUsesT<SCHEME>) win[n].T = tile[5][ty][tx + n];
+        if constexpr (SENS) {
+          if (n == St::HW - 1 || n == St::HW) win[n].s = tile[7][ty][tx + n];
+        }
       }
-      if constexpr (VISC) {
+      face_conv<R, 0, ORDER, SCHEME, VISC, VO>(c, tvd, variant, win, F);
+    }
+    if constexpr (VISC) {
+      OX_SPLIT_FENCE();
+      CD win[St::SW];
+#pragma unroll
+      for (int n = 0; n < St::SW; ++n) {
+        win[n].u = tile[1][ty][tx + n]; win[n].v = tile[2][ty][tx + n]; win[n].w = tile[3][ty][tx + n];
+        win[n].T = tile[5][ty][tx + n];
         win[n].mu = tile[6][ty][tx + n];
         win[n].d1 = tile[8][ty][tx + n]; win[n].d2 = tile[9][ty][tx + n];
         win[n].d3 = tile[10][ty][tx + n]; win[n].d4 = tile[11][ty][tx + n];
         if constexpr (VISC == 2) { win[n].mt = tile[NF - 2][ty][tx + n]; win[n].H = tile[NF - 1][ty][tx + n]; }
       }
+      face_visc<R, 0, ORDER, SCHEME, VISC, VO>(c, R(g.inv_dx), win, F);
     }
-    face_flux<R, 0, ORDER, SCHEME, VISC, VO>(c, tvd, variant, R(g.inv_dx), win, F);
   }
   double Fr[5];
 #pragma unroll

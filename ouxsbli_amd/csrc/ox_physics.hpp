@@ -148,14 +148,21 @@ OX_HD void keep_face(const PhysC<R>& c, const R* rho, const R* u, const R* v, co
 // ---------------------------------------------------------------------------
 // MUSCL.  src/calc_muscl.f90:19-169.  tvd: OX_TVD_NONE / TVD / HYBRID.
 // ---------------------------------------------------------------------------
-// MIN/MAX of the limiters: the select form in FP64, where the results must
-// match the FP64 oracle bit for bit, and the hardware min/max (one
-// instruction, no compare-and-select) in FP32, which differs from it only in
-// the sign of a zero result -- and a limiter's zero slope adds nothing either way.
+// MIN/MAX of the limiters: the hardware min/max (v_min_f64 / v_min_f32, one
+// instruction where the compare-and-select form takes three).  For every pair
+// of non-NaN operands it returns the value the select form (b < a ? b : a)
+// returns; the two differ only in the sign of a zero when the operands are
+// +0 and -0.  A limiter's result enters the interface state as an addend
+// (a + 0 and a - 0 are a either way) or, through fsign1 in minmod, as the
+// factor of a zero, so the sign of a zero never reaches a nonzero value:
+// the FP64 results equal the oracle's select-form results value for value.
+// Other value types (the operation counter) keep the select form.
 template <class R> OX_HD R lmin(R a, R b) { return fmin2(a, b); }
 template <class R> OX_HD R lmax(R a, R b) { return fmax2(a, b); }
 template <> OX_HD float lmin<float>(float a, float b) { return __builtin_fminf(a, b); }
 template <> OX_HD float lmax<float>(float a, float b) { return __builtin_fmaxf(a, b); }
+template <> OX_HD double lmin<double>(double a, double b) { return __builtin_fmin(a, b); }
+template <> OX_HD double lmax<double>(double a, double b) { return __builtin_fmax(a, b); }
 
 template <class R> OX_HD R minmod2(R x, R y) {
   const R sgn = fsign1(x);

@@ -67,9 +67,9 @@ def kernel_model(spec, ops):
 # after a kernel change.
 MEASURED_TRAFFIC_256 = {
     "prim": 1957e6,
-    "sweep_x": 2950e6,
-    "sweep_y": 3399e6,
-    "sweep_z_rk": (5426e6 + 2 * 6268e6) / 3,
+    "sweep_x": 2282e6,
+    "sweep_y": 2671e6,
+    "sweep_z_rk": (4746e6 + 2 * 5524e6) / 3,
     "ghost_fill": 175e6,
 }
 
@@ -122,17 +122,6 @@ def les_leg(n, steps):
     out["fp32_bulk"]["ke_rel_err_vs_fp64"] = abs(k32 - k64) / abs(k64)
     out["fp32_bulk"]["speedup_vs_fp64"] = round(out["fp32_bulk"]["Mcell_updates_per_s_stage"] /
                                                 out["fp64"]["Mcell_updates_per_s_stage"], 3)
-    # the fused stage kernel against both roofs: compulsory FP64 state bytes and
-    # the algorithmic operation count of smooth faces (ouxsbli_amd/flops/opcount.json;
-    # limited faces cost more, so this understates the work done)
-    ops = _opcounts()["roe6_hybrid_2d_rk3"]["per_cell_stage_smooth"]
-    rate = out["fp32_bulk"]["device_busy_Mcell_updates_per_s_stage"] * 1e6
-    out["fp32_bulk"]["roofline"] = {
-        "kernel": "k2d_stage<float,6,ROE,*>", "bound": "hbm", "unit": "GB/s",
-        "achieved": round(rate * OS_BYTES_PER_CELL_STAGE / 1e9, 1), "peak": HBM_PEAK_GBS,
-        "frac": round(rate * OS_BYTES_PER_CELL_STAGE / 1e9 / HBM_PEAK_GBS, 4),
-        "ops_per_cell_stage": ops, "achieved_TFLOPs": round(rate * ops / 1e12, 2), "peak_TFLOPs": FP32_PEAK_TFLOPS,
-        "flop_frac": round(rate * ops / 1e12 / FP32_PEAK_TFLOPS, 4)}
     out["state_ok"] = bool(res["fp32_bulk"]["ok"] and res["fp64"]["ok"])
     return out
 
@@ -164,7 +153,7 @@ def batched_os_leg(ncase, steps, unbatched_steps, cpu_seconds):
     advanced together: one persistent launch per RK stage over the tiles of
     all cases, replayed from a HIP graph.  Timed beside: the FP64 plain form
     of the same batch, and the FP32 cases advanced one by one (one launch per
-    stage per case)."""
+    stage per case), in both forms."""
     import numpy as np
     from ouxsbli_amd import cases2d as c2
     from ouxsbli_amd.batch import Batch
@@ -214,18 +203,23 @@ def batched_os_leg(ncase, steps, unbatched_steps, cpu_seconds):
         errs.append(float(np.sqrt(np.mean(d * d) / np.mean(a[:, 0, :] ** 2))))
     out["fp32_bulk"]["rho_rel_l2_vs_fp64_max"] = max(errs)
     out["fp32_bulk"]["rho_rel_l2_vs_fp64_median"] = float(np.median(errs))
-    # the same FP32 cases one at a time (one launch per stage per case)
-    specs = c2.config4_specs(ncase, precision="fp32_bulk")
-    t_dev = 0.0
-    cs1 = 0
-    for s in specs:
-        with Solver2D(s) as h:
-            h.set_state(c2.os_state(s))
-            h.step(unbatched_steps)
-            t_dev += h.last_step_ms() * 1e-3
-        cs1 += s.interior_cells * 3 * unbatched_steps
-    out["fp32_bulk_one_case_at_a_time"] = {"device_Mcell_updates_per_s_stage": round(cs1 / t_dev / 1e6, 1),
-                                           "steps": unbatched_steps}
+    # the same cases one at a time through the plain single-case kernels (one
+    # launch per stage per case), in both forms
+    for prec in ("fp32_bulk", "fp64"):
+        specs = c2.config4_specs(ncase, precision=prec)
+        t_dev = 0.0
+        cs1 = 0
+        for s in specs:
+            with Solver2D(s) as h:
+                h.set_state(c2.os_state(s))
+                h.step(unbatched_steps)
+                t_dev += h.last_step_ms() * 1e-3
+            cs1 += s.interior_cells * 3 * unbatched_steps
+        out[f"{prec}_one_case_at_a_time"] = {"device_Mcell_updates_per_s_stage": round(cs1 / t_dev / 1e6, 1),
+                                             "steps": unbatched_steps}
+    out["fp32_bulk"]["speedup_vs_fp64_one_case_at_a_time"] = round(
+        out["fp32_bulk"]["device_busy_Mcell_updates_per_s_stage"] /
+        out["fp64_one_case_at_a_time"]["device_Mcell_updates_per_s_stage"], 3)
     if cpu_seconds > 0:
         sys.path.insert(0, str(ROOT / "tests"))
         import oracle_lib

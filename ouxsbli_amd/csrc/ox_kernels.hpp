@@ -446,6 +446,9 @@ struct TileMap {
 #ifndef OX_XCD_SWIZZLE
 #define OX_XCD_SWIZZLE 1
 #endif
+#ifndef OX_XSWEEP_ZMAJOR
+#define OX_XSWEEP_ZMAJOR 1
+#endif
 __device__ __forceinline__ int xcd_tile(int b, int n) {
   if (!OX_XCD_SWIZZLE || (n & 7)) return b;
   return (b & 7) * (n >> 3) + (b >> 3);
@@ -458,6 +461,25 @@ __device__ __forceinline__ void launch_tile(int& bx, int& by, int& bz) {
   bx = t % gx;
   by = (t / gx) % gy;
   bz = t / (gx * gy);
+}
+
+// The x sweep's tiles (bx, row band by, plane bz) in x, z, y order under
+// xcd_tile: one XCD's run covers a band of rows through many planes, so both
+// the y and the z neighbours a row tile's cross-derivative stencils read
+// (rows by +- 1, planes bz +- 3) are fetched by tiles resident at about the
+// same time in one L2.
+__device__ __forceinline__ void launch_tile_xzy(int& bx, int& by, int& bz) {
+  const int gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+  const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gz);
+#if OX_XSWEEP_ZMAJOR
+  bx = t % gx;
+  bz = (t / gx) % gz;
+  by = t / (gx * gz);
+#else
+  bx = t % gx;
+  by = (t / gx) % gy;
+  bz = t / (gx * gy);
+#endif
 }
 
 // Case owning global tile t (map[].start ascending, map[0].start == 0).
@@ -692,7 +714,7 @@ template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
 __global__ __launch_bounds__(256) void k_sweep_x(SweepArgs<R, S> a) {
   __shared__ R tile[xsweep_nf<VISC>()][XROWS][64];
   int bx, by, bz;
-  launch_tile(bx, by, bz);
+  launch_tile_xzy(bx, by, bz);
   sweep_x_tile<R, S, ORDER, SCHEME, VISC, VO>(a, bx, by, bz, tile);
 }
 

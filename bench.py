@@ -261,6 +261,24 @@ def serving_leg(n_requests, fp64_value):
     out["slots"] = SERVE_SLOTS
     busy = s["device_busy_Mcell_updates_per_s_stage"]
     out["device_busy_vs_fp64_plain"] = round(busy / fp64_value, 3) if busy else None
+    # the same requests through the plain FP64 single-case path, one at a time
+    # (1 warm-up step, then a sample of timed steps each, device time)
+    import dataclasses
+    from ouxsbli_amd.solver import Solver
+    sample = 10
+    cs = 0
+    dev_s = 0.0
+    for r in serve.khi_stream(n_requests, rate_hz=20.0, nsteps=200, precision="fp32_bulk"):
+        spec = dataclasses.replace(r.spec, precision="fp64")
+        with Solver(spec) as h:
+            h.set_state(r.Q0)
+            h.step(1)
+            dev_s += h.step(sample) * 1e-3
+        cs += spec.interior_cells * spec.stages * sample
+    plain = cs / dev_s / 1e6
+    out["fp64_plain_same_requests"] = {"device_Mcell_updates_per_s_stage": round(plain, 1), "steps_per_request": sample,
+                                       "form": "each request alone through the FP64 single-case path"}
+    out["device_busy_vs_fp64_plain_same_requests"] = round(busy / plain, 3) if busy else None
     return out
 
 

@@ -214,6 +214,7 @@ int build_plan(ox_batch* b) {
         m.start = nt[d];
         m.nbx = (int)nbx;
         m.nby = (int)nby;
+        m.nbz = (int)nbz;
         nt[d] += (int)(nbx * nby * nbz);
         maps.push_back(m);
       }

@@ -433,8 +433,8 @@ template <class R, class S> struct SweepArgs {
 
 struct TileMap {
   int start;     // first global tile of the case
-  int nbx, nby;  // tiles along the first two launch dimensions
-  int pad;
+  int nbx, nby;  // tiles along the three launch dimensions
+  int nbz;
 };
 
 // Workgroups are dispatched round-robin over the eight XCDs, and each XCD has
@@ -725,9 +725,16 @@ __global__ __launch_bounds__(256) void k_sweep_x_batch(const SweepArgs<R, S>* __
   __shared__ R tile[xsweep_nf<VISC>()][XROWS][64];
   for (int t = xcd_tile(blockIdx.x, gridDim.x); t < ntiles; t += gridDim.x) {
     const int ci = find_case(map, ncase, t);
-    const int local = t - map[ci].start, nbx = map[ci].nbx, nby = map[ci].nby;
+    const int local = t - map[ci].start, nbx = map[ci].nbx;
+#if OX_XSWEEP_ZMAJOR
+    const int nbz = map[ci].nbz;
+    sweep_x_tile<R, S, ORDER, SCHEME, VISC, VO>(cases[ci], local % nbx, local / (nbx * nbz), (local / nbx) % nbz,
+                                                tile);
+#else
+    const int nby = map[ci].nby;
     sweep_x_tile<R, S, ORDER, SCHEME, VISC, VO>(cases[ci], local % nbx, (local / nbx) % nby, local / (nbx * nby),
                                                 tile);
+#endif
     __syncthreads();  // the next tile overwrites the LDS rows
   }
 }

@@ -248,15 +248,18 @@ def serving_leg(n_requests, fp64_value):
     """Kelvin-Helmholtz serving stream through the batch engine (FP32-bulk):
     seeded Poisson arrivals at 20 req/s, interior grids 64^3..256^3 (multiples
     of 16), KEEP or SLAU at ORDER 4 or 6, Euler, RK3, 200 steps each; FIFO
-    admission by HBM footprint (64 GiB budget) and 16 slots."""
+    admission by HBM footprint (64 GiB budget) and 16 slots, each request's
+    initial state built on the device when it is admitted."""
     from ouxsbli_amd import serve
-    reqs = serve.khi_stream(n_requests, rate_hz=20.0, nsteps=200, precision="fp32_bulk")
+    # requests carry parameters only: each initial state is built on the device at admission
+    reqs = serve.khi_stream(n_requests, rate_hz=20.0, nsteps=200, precision="fp32_bulk", with_states=False)
     rep = serve.serve(reqs, hbm_budget=SERVE_BUDGET, max_cases=SERVE_SLOTS, keep_results=False)
     s = rep.summary()
     out = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()}
     out["workload"] = ("3-D Kelvin-Helmholtz stream: double tanh shear layer d=L/28, Ma 0.5, 1e-3 U seeded noise, "
                        f"{n_requests} requests, Poisson 20 req/s, grids 64^3-256^3, KEEP/SLAU ORDER 4/6, Euler RK3, "
-                       "200 steps each, FP32-bulk, batched per stage with HIP graph replay")
+                       "200 steps each, FP32-bulk, initial states built on the device at admission, batched per stage "
+                       "with HIP graph replay")
     out["hbm_budget_GiB"] = SERVE_BUDGET >> 30
     out["slots"] = SERVE_SLOTS
     busy = s["device_busy_Mcell_updates_per_s_stage"]
@@ -268,10 +271,10 @@ def serving_leg(n_requests, fp64_value):
     sample = 10
     cs = 0
     dev_s = 0.0
-    for r in serve.khi_stream(n_requests, rate_hz=20.0, nsteps=200, precision="fp32_bulk"):
+    for r in serve.khi_stream(n_requests, rate_hz=20.0, nsteps=200, precision="fp32_bulk", with_states=False):
         spec = dataclasses.replace(r.spec, precision="fp64")
         with Solver(spec) as h:
-            h.set_state(r.Q0)
+            h.init_state()
             h.step(1)
             dev_s += h.step(sample) * 1e-3
         cs += spec.interior_cells * spec.stages * sample

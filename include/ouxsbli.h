@@ -331,8 +331,49 @@ int ox2d_last_step_ms(ox2d_solver* s, double* ms);
 int ox2d_footprint(const ox2d_config* cfg, uint64_t* bytes);
 
 /* Admits a 2-D case into a batch; like ox_batch_admit.  ox_batch_retire of its
- * id copies out Q(nx,4,ny). */
+ * id copies out Q(nx,4,ny).  Q == NULL builds the oblique-shock initial state
+ * on the device instead (ox2d_init_state). */
 int ox_batch_admit2d(ox_batch* b, const ox2d_config* cfg, const double* Q, int32_t nsteps, int64_t* id);
+
+/* ---------------------------------------------------------------------------
+ * Initial states built on the device.  The case set_init routines sample the
+ * initial field at the cell centres of the interior on the host, then the
+ * host fills the ghosts and pre_calc divides by the Jacobian
+ * (3D_solver/NSTGV/set.f90:16-47, 3D_solver/src/set_bc_common.f90:11-97,
+ * 3D_solver/src/preprocess.f90.fypp:65-71; 2D_solver/OS/set.f90:28-46).
+ * These entry points do all of it in one kernel plus the ghost fill, with no
+ * host array and no upload, so a server can admit a request from its
+ * parameters alone.
+ * ------------------------------------------------------------------------- */
+#define OX_IC_TAYLOR_GREEN 1 /* NSTGV set_init: p, rho, u, v (w = 0) at the cell centres            */
+#define OX_IC_SHEAR_LAYER 2  /* double tanh shear layer in y with seeded velocity noise (no reference
+                                counterpart: the Kelvin-Helmholtz request stream's field)           */
+
+typedef struct ox_ic {
+  int32_t kind;     /* OX_IC_*                                                                  */
+  int32_t reserved; /* 0                                                                        */
+  uint64_t seed;    /* shear layer: noise seed                                                  */
+  double L0;        /* Taylor-Green: length scale of sin(x/L0), cos(2x/L0), ...                 */
+  double V0;        /* Taylor-Green: V0; shear layer: U, the velocity difference across a layer */
+  double rho0;      /* Taylor-Green: RHO0 of the pressure amplitude; shear layer: density       */
+  double p0;        /* Taylor-Green: mean pressure; shear layer: pressure                       */
+  double T0;        /* Taylor-Green: temperature, rho = p / (R T0)                              */
+  double delta;     /* shear layer: thickness d of tanh((y - Ly/4) / d)                         */
+  double noise;     /* shear layer: amplitude of the uniform [-1, 1) velocity noise             */
+} ox_ic;
+
+/* Builds the initial state of `ic` in the handle (interior at the cell
+ * centres of ox_create's grid, then the periodic ghost fill), replacing
+ * ox_set_state of a host array. */
+int ox_init_state(ox_solver* s, const ox_ic* ic);
+
+/* ox_batch_admit with the initial state built on the device from `ic`. */
+int ox_batch_admit_ic(ox_batch* b, const ox_config* cfg, const ox_ic* ic, int32_t nsteps, int64_t* id);
+
+/* The oblique-shock initial state of the configuration (its inflow state
+ * everywhere, its post-shock state on the top row from top_split on),
+ * replacing ox2d_set_state of a host array (2D_solver/OS/set.f90:28-46). */
+int ox2d_init_state(ox2d_solver* s);
 
 /* Thread-local text of the last error on this thread ("" if none). */
 const char* ox_last_error(void);

@@ -86,6 +86,27 @@ class Ox2dConfig(ctypes.Structure):
     ]
 
 
+OX_IC_TAYLOR_GREEN = 1
+OX_IC_SHEAR_LAYER = 2
+
+
+class OxIc(ctypes.Structure):
+    """``ox_ic`` -- an initial condition built on the device."""
+
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("seed", ctypes.c_uint64),
+        ("L0", ctypes.c_double),
+        ("V0", ctypes.c_double),
+        ("rho0", ctypes.c_double),
+        ("p0", ctypes.c_double),
+        ("T0", ctypes.c_double),
+        ("delta", ctypes.c_double),
+        ("noise", ctypes.c_double),
+    ]
+
+
 class OxDiag(ctypes.Structure):
     """``ox_diag`` -- interior diagnostics."""
 
@@ -180,6 +201,9 @@ EXPORTS = (
     "ox2d_last_step_ms",
     "ox2d_footprint",
     "ox_batch_admit2d",
+    "ox_init_state",
+    "ox_batch_admit_ic",
+    "ox2d_init_state",
     "ox_last_error",
     "ox_version",
 )

@@ -97,6 +97,13 @@ def load() -> ctypes.CDLL:
     L.ox_batch_admit2d.argtypes = [P, ctypes.POINTER(_abi.Ox2dConfig), D, ctypes.c_int32,
                                    ctypes.POINTER(ctypes.c_int64)]
     L.ox_batch_admit2d.restype = I
+    L.ox_init_state.argtypes = [P, ctypes.POINTER(_abi.OxIc)]
+    L.ox_init_state.restype = I
+    L.ox_batch_admit_ic.argtypes = [P, ctypes.POINTER(_abi.OxConfig), ctypes.POINTER(_abi.OxIc), ctypes.c_int32,
+                                    ctypes.POINTER(ctypes.c_int64)]
+    L.ox_batch_admit_ic.restype = I
+    L.ox2d_init_state.argtypes = [P]
+    L.ox2d_init_state.restype = I
     L.ox_last_error.argtypes = []
     L.ox_last_error.restype = ctypes.c_char_p
     L.ox_version.argtypes = []

@@ -12,6 +12,7 @@ HBM budget) and leave when their step count is reached (``done`` /
 from __future__ import annotations
 
 import ctypes
+from typing import Optional
 
 import numpy as np
 
@@ -67,19 +68,32 @@ class Batch:
         except Exception:
             pass
 
-    def admit(self, spec, Q: np.ndarray, nsteps: int) -> int:
+    def admit(self, spec, Q: Optional[np.ndarray], nsteps: int) -> int:
         """Admits a case for nsteps RK steps and returns its id: a 3-D
         CaseSpec with physical state Q of numpy shape (nz, ny, 5, nx), or a
-        2-D CaseSpec2D with Q of shape (ny, 4, nx).  Raises Busy if it does
-        not fit now."""
-        Q = np.ascontiguousarray(Q, dtype=np.float64)
-        if Q.shape != spec.state_shape():
-            raise ValueError(f"state shape {Q.shape} != {spec.state_shape()}")
+        2-D CaseSpec2D with Q of shape (ny, 4, nx).  Q None builds the case's
+        initial state on the device instead (CaseSpec.device_ic, or the
+        oblique-shock state of a 2-D case), so nothing is uploaded.  Raises
+        Busy if it does not fit now."""
         cfg = spec.to_ox()
         cid = ctypes.c_int64()
-        fn = self._lib.ox_batch_admit2d if getattr(spec, "dim", 3) == 2 else self._lib.ox_batch_admit
-        rc = fn(self._h, ctypes.byref(cfg), Q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(nsteps),
-                ctypes.byref(cid))
+        two_d = getattr(spec, "dim", 3) == 2
+        if Q is None:
+            if two_d:
+                rc = self._lib.ox_batch_admit2d(self._h, ctypes.byref(cfg), None, int(nsteps), ctypes.byref(cid))
+            else:
+                ic = spec.device_ic()
+                if ic is None:
+                    raise ValueError(f"no device initial condition for ic {spec.ic!r}")
+                rc = self._lib.ox_batch_admit_ic(self._h, ctypes.byref(cfg), ctypes.byref(ic), int(nsteps),
+                                                 ctypes.byref(cid))
+        else:
+            Q = np.ascontiguousarray(Q, dtype=np.float64)
+            if Q.shape != spec.state_shape():
+                raise ValueError(f"state shape {Q.shape} != {spec.state_shape()}")
+            fn = self._lib.ox_batch_admit2d if two_d else self._lib.ox_batch_admit
+            rc = fn(self._h, ctypes.byref(cfg), Q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(nsteps),
+                    ctypes.byref(cid))
         if rc == _abi.OX_EBUSY:
             raise Busy(self._lib.ox_last_error().decode())
         _native.check(rc)

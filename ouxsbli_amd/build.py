@@ -41,6 +41,7 @@ UNITS = {
     "ox_fp32.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "ox_fp32_batch.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "ox2d_capi.hip": ["-ffp-contract=off"],
+    "ox_ic.hip": ["-ffp-contract=off"],
     "ox2d_fp64.hip": ["-ffp-contract=off"],
     "ox2d_fp32.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }

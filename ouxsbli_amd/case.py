@@ -25,9 +25,9 @@ import numpy as np
 
 from . import io_vtr
 from .cases import CaseSpec, cell_centres
-from .cases2d import CaseSpec2D, os_state
+from .cases2d import CaseSpec2D
 from .fortran_params import read_fypp_macros, read_parameters
-from .ic import bc_to_params, ic_to_params, initial_state
+from .ic import bc_to_params, device_ic, ic_to_params, initial_state
 from .patcher import _is_macro_line, patch
 
 _ALIAS = {
@@ -172,10 +172,10 @@ class Case:
         recal.mkdir(exist_ok=True)
         nt, np_ = int(g.get("nt", 1)), int(g.get("np", 1))
         offset = int(g.get("step_offset", 0))
+        # a fresh run builds OS set_init on the device; a restart uploads its record
+        Q = None
         if bool(self.macros.get("RESTART", False)):
             Q = io_vtr.read_restart(recal / "Q00001.dat", spec.state_shape())
-        else:
-            Q = os_state(spec)
         # set_grid (OS/set.f90:8-26): running sums of the uniform spacing
         x = np.zeros(spec.nx)
         y = np.zeros(spec.ny)
@@ -186,7 +186,10 @@ class Case:
             y[j] = y[j - 1] + dy0
         z = np.zeros(1)
         with Solver2D(spec) as s:
-            s.set_state(Q)
+            if Q is None:
+                s.init_state()
+            else:
+                s.set_state(Q)
 
             def output(step):
                 rho, p, vel = io_vtr.primitives_for_print2d(s.get_state(), spec.gamma)
@@ -221,17 +224,25 @@ class Case:
         (self.workdir / "recal").mkdir(exist_ok=True)
         nt, np_ = int(g.get("nt", 1)), int(g.get("np", 1))
         restart = bool(self.macros.get("RESTART", False))
+        # the case's set_init: on the device where the library has it (no host
+        # array, no upload), else on the host; a restart uploads its record
+        Q, dev_ic = None, None
         if restart:
             Q = io_vtr.read_restart(self.workdir / "recal" / "Q00001.dat", spec.state_shape())
         else:
-            Q = initial_state(self.source.name, spec, g)
+            dev_ic = device_ic(self.source.name, spec, g)
+            if dev_ic is None:
+                Q = initial_state(self.source.name, spec, g)
         x = cell_centres(spec.nx, spec.Lx, spec.order)
         y = cell_centres(spec.ny, spec.Ly, spec.order)
         z = cell_centres(spec.nz, spec.Lz, spec.order)
         diag = io_vtr.Diagnostics(data)
         offset = int(g.get("step_offset", 0))
         with Solver(spec) as s:
-            s.set_state(Q)
+            if dev_ic is not None:
+                s.init_state(dev_ic)
+            else:
+                s.set_state(Q)
             jac = self.jacobian()
 
             def output(step):

@@ -105,6 +105,17 @@ class CaseSpec:
         c.Lx, c.Ly, c.Lz = self.Lx, self.Ly, self.Lz
         return c
 
+    def device_ic(self):
+        """The ``ox_ic`` that builds this case's initial state on the device
+        (ox_init_state), or None where only the host builds it."""
+        if self.ic == "tgv":
+            return _abi.OxIc(kind=_abi.OX_IC_TAYLOR_GREEN, L0=self.L0, V0=self.V0, rho0=self.RHO0, p0=self.p0,
+                             T0=self.T0)
+        if self.ic == "khi_tanh":
+            return _abi.OxIc(kind=_abi.OX_IC_SHEAR_LAYER, seed=int(self.seed), V0=self.V0, rho0=self.RHO0,
+                             p0=self.p0, delta=self.Ly / 28.0, noise=1e-3 * self.V0)
+        return None
+
     def state_shape(self):
         """numpy shape of Q(nx,5,ny,nz) in C order (z, y, var, x)."""
         return (self.nz, self.ny, 5, self.nx)
@@ -232,23 +243,52 @@ def khi_spec(n: int, order: int, scheme: str, precision: str = "fp64", seed: int
     return spec
 
 
+_SM_GAMMA = 0x9E3779B97F4A7C15
+_U64 = (1 << 64) - 1
+
+
+def splitmix64(z: np.ndarray) -> np.ndarray:
+    """SplitMix64's output function (Steele, Lea and Flood, OOPSLA 2014) on a
+    uint64 array: the counter plus the golden-ratio increment, then two
+    xor-shift-multiply rounds and a final xor-shift, all modulo 2^64."""
+    with np.errstate(over="ignore"):
+        z = z + np.uint64(_SM_GAMMA)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def cell_noise(seed: int, shape, field: int) -> np.ndarray:
+    """Uniform noise in [-1, 1) of velocity component `field` (0, 1, 2) at
+    every cell of the padded (nz, ny, nx) array: SplitMix64 of the key
+    (seed + 1) * 0x9E3779B97F4A7C15 + 3 m + field for linear cell m, whose top
+    53 bits are scaled to [0, 1) exactly.  The device kernel of
+    ox_init_state (ouxsbli_amd/csrc/ox_ic.hip) draws the same numbers."""
+    base = ((int(seed) + 1) * _SM_GAMMA) & _U64
+    m = np.arange(int(np.prod(shape)), dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        key = np.uint64(base) + (np.uint64(3) * m + np.uint64(field))
+    u01 = (splitmix64(key) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    return (2.0 * u01 - 1.0).reshape(shape)
+
+
 def khi_state(spec: CaseSpec) -> np.ndarray:
     """Double tanh shear layer u(y) = U/2 * (tanh((y-1/4)/d) - tanh((y-3/4)/d) - 1),
     d = L/28, plus seeded noise of amplitude 1e-3 U in u, v, w (uniform in
-    [-1, 1], drawn in that order over the whole (z, y, x) array); uniform
-    density and pressure."""
+    [-1, 1), a counter-based draw per cell and component, see cell_noise);
+    uniform density and pressure.  The host form of the OX_IC_SHEAR_LAYER
+    initial condition the library builds on the device."""
     y = cell_centres(spec.ny, spec.Ly, spec.order)
     U, d = spec.V0, spec.Ly / 28.0
-    rng = np.random.default_rng(spec.seed)
     shape = (spec.nz, spec.ny, spec.nx)
     prof = 0.5 * U * (np.tanh((y - 0.25 * spec.Ly) / d) - np.tanh((y - 0.75 * spec.Ly) / d) - 1.0)
     amp = 1e-3 * U
-    u = rng.uniform(-1.0, 1.0, shape)
+    u = cell_noise(spec.seed, shape, 0)
     u *= amp
     u += prof[None, :, None]
-    v = rng.uniform(-1.0, 1.0, shape)
+    v = cell_noise(spec.seed, shape, 1)
     v *= amp
-    w = rng.uniform(-1.0, 1.0, shape)
+    w = cell_noise(spec.seed, shape, 2)
     w *= amp
     rho, p = spec.RHO0, spec.p0
     Q = np.empty(spec.state_shape(), dtype=np.float64)

@@ -394,6 +394,59 @@ void free_case(BCase* c) {
 
 }  // namespace
 
+// One 3-D case joins the batch; `init` gives its new handle the initial
+// state (a host upload or a device initial condition).
+namespace {
+template <class Init>
+int admit3d(ox_batch* b, const ox_config* cfg, Init init, int32_t nsteps, int64_t* id) {
+  if (nsteps < 1) return ox_fail(OX_EINVAL, "nsteps must be >= 1");
+  if (cfg->device != b->cfg.device) return ox_fail(OX_EINVAL, "case device differs from the batch device");
+  if (cfg->visc == OX_VISC_LES) return ox_fail(OX_EUNSUPPORTED, "LES cases are not batched (use ox_step)");
+  const size_t need = solver_footprint(*cfg);
+  if ((int)(b->active.size() + b->done.size()) >= b->cfg.max_cases) return ox_fail(OX_EBUSY, "no free slot");
+  if (b->bytes_in_use + need > b->cfg.hbm_budget) return ox_fail(OX_EBUSY, "footprint exceeds the free HBM budget");
+  ox_solver* s = nullptr;
+  int rc = ox_create(cfg, &s);
+  if (rc != OX_OK) return rc;
+  rc = init(s);
+  if (rc == OX_OK && b->cfg.autotune) rc = tune_case(b, s);
+  if (rc != OX_OK) {
+    ox_destroy(s);
+    return rc;
+  }
+  // primitives of the initial state on the handle's own stream, then the
+  // case joins the engine's stream
+  {
+    ox::BatchOps ops;
+    const bool ok = cfg->precision == OX_PREC_FP64
+                        ? ox::find_batch_fp64(cfg->order, cfg->scheme, cfg->visc, s->C.vorder, &ops)
+                        : ox::find_batch_fp32(cfg->order, cfg->scheme, cfg->visc, s->C.vorder, &ops);
+    if (!ok) {
+      ox_destroy(s);
+      return ox_fail(OX_EUNSUPPORTED, "no batched specialisation for this case");
+    }
+    if (s->C.prim_for != s->QJ) ops.prepare(s->C, s->QJ);  // k_prim
+    if (hipStreamSynchronize(s->C.stream) != hipSuccess) {
+      ox_destroy(s);
+      return ox_fail(OX_EDEVICE, "case initialisation failed");
+    }
+  }
+  s->C.stream = b->stream;
+  s->C.prof = false;
+  auto c = std::make_unique<BCase>();
+  c->id = b->next_id++;
+  c->s = s;
+  c->remaining = nsteps;
+  c->bytes = need;
+  c->group = -1;
+  b->bytes_in_use += need;
+  *id = c->id;
+  b->active.push_back(std::move(c));
+  invalidate(b);
+  return OX_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int ox_footprint(const ox_config* cfg, uint64_t* bytes) {
@@ -485,51 +538,12 @@ void ox_batch_destroy(ox_batch* b) {
 
 int ox_batch_admit(ox_batch* b, const ox_config* cfg, const double* Q, int32_t nsteps, int64_t* id) {
   if (!b || !cfg || !Q || !id) return ox_fail(OX_EINVAL, "null argument");
-  if (nsteps < 1) return ox_fail(OX_EINVAL, "nsteps must be >= 1");
-  if (cfg->device != b->cfg.device) return ox_fail(OX_EINVAL, "case device differs from the batch device");
-  if (cfg->visc == OX_VISC_LES) return ox_fail(OX_EUNSUPPORTED, "LES cases are not batched (use ox_step)");
-  const size_t need = solver_footprint(*cfg);
-  if ((int)(b->active.size() + b->done.size()) >= b->cfg.max_cases) return ox_fail(OX_EBUSY, "no free slot");
-  if (b->bytes_in_use + need > b->cfg.hbm_budget) return ox_fail(OX_EBUSY, "footprint exceeds the free HBM budget");
-  ox_solver* s = nullptr;
-  int rc = ox_create(cfg, &s);
-  if (rc != OX_OK) return rc;
-  rc = ox_set_state(s, Q);
-  if (rc == OX_OK && b->cfg.autotune) rc = tune_case(b, s);
-  if (rc != OX_OK) {
-    ox_destroy(s);
-    return rc;
-  }
-  // primitives of the initial state on the handle's own stream, then the
-  // case joins the engine's stream
-  {
-    ox::BatchOps ops;
-    const bool ok = cfg->precision == OX_PREC_FP64
-                        ? ox::find_batch_fp64(cfg->order, cfg->scheme, cfg->visc, s->C.vorder, &ops)
-                        : ox::find_batch_fp32(cfg->order, cfg->scheme, cfg->visc, s->C.vorder, &ops);
-    if (!ok) {
-      ox_destroy(s);
-      return ox_fail(OX_EUNSUPPORTED, "no batched specialisation for this case");
-    }
-    if (s->C.prim_for != s->QJ) ops.prepare(s->C, s->QJ);  // k_prim
-    if (hipStreamSynchronize(s->C.stream) != hipSuccess) {
-      ox_destroy(s);
-      return ox_fail(OX_EDEVICE, "case initialisation failed");
-    }
-  }
-  s->C.stream = b->stream;
-  s->C.prof = false;
-  auto c = std::make_unique<BCase>();
-  c->id = b->next_id++;
-  c->s = s;
-  c->remaining = nsteps;
-  c->bytes = need;
-  c->group = -1;
-  b->bytes_in_use += need;
-  *id = c->id;
-  b->active.push_back(std::move(c));
-  invalidate(b);
-  return OX_OK;
+  return admit3d(b, cfg, [Q](ox_solver* s) { return ox_set_state(s, Q); }, nsteps, id);
+}
+
+int ox_batch_admit_ic(ox_batch* b, const ox_config* cfg, const ox_ic* ic, int32_t nsteps, int64_t* id) {
+  if (!b || !cfg || !ic || !id) return ox_fail(OX_EINVAL, "null argument");
+  return admit3d(b, cfg, [ic](ox_solver* s) { return ox_init_state(s, ic); }, nsteps, id);
 }
 
 int ox_batch_tune(ox_batch* b, const ox_config* cfg, const double* Q) {
@@ -544,7 +558,7 @@ int ox_batch_tune(ox_batch* b, const ox_config* cfg, const double* Q) {
 }
 
 int ox_batch_admit2d(ox_batch* b, const ox2d_config* cfg, const double* Q, int32_t nsteps, int64_t* id) {
-  if (!b || !cfg || !Q || !id) return ox_fail(OX_EINVAL, "null argument");
+  if (!b || !cfg || !id) return ox_fail(OX_EINVAL, "null argument");
   if (nsteps < 1) return ox_fail(OX_EINVAL, "nsteps must be >= 1");
   if (cfg->device != b->cfg.device) return ox_fail(OX_EINVAL, "case device differs from the batch device");
   const size_t need = solver2d_footprint(*cfg);
@@ -553,7 +567,7 @@ int ox_batch_admit2d(ox_batch* b, const ox2d_config* cfg, const double* Q, int32
   ox2d_solver* s = nullptr;
   int rc = ox2d_create(cfg, &s);
   if (rc != OX_OK) return rc;
-  rc = ox2d_set_state(s, Q);
+  rc = Q ? ox2d_set_state(s, Q) : ox2d_init_state(s);
   if (rc != OX_OK) {
     ox2d_destroy(s);
     return rc;

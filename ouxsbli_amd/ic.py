@@ -181,6 +181,19 @@ def etgv_state(spec, g):
 CASE_STATES = {"NSTGV": nstgv_state, "ETGV": etgv_state, "KHI": khi_state}
 
 
+def device_ic(case_name: str, spec, g: dict):
+    """The ``ox_ic`` that builds the case's set_init on the device, or None
+    where only the host form exists.  NSTGV's set_init (3D_solver/NSTGV/
+    set.f90:16-47) is the library's OX_IC_TAYLOR_GREEN with the case file's
+    l0, v0, rho0, p0 and t; an explicit ic= of the Case, and every other
+    case, stay on the host."""
+    if g.get("ic_type") is not None or case_name != "NSTGV":
+        return None
+    spec.ic = "tgv"
+    spec.L0, spec.V0, spec.RHO0, spec.p0, spec.T0 = g["l0"], g["v0"], g["rho0"], g["p0"], g["t"]
+    return spec.device_ic()
+
+
 def initial_state(case_name: str, spec, g: dict) -> np.ndarray:
     ic_type = g.get("ic_type")
     if ic_type == IC_UNIFORM:

@@ -35,7 +35,7 @@ class Request:
     spec: CaseSpec
     nsteps: int
     arrival: float                       # seconds after the stream starts
-    Q0: Optional[np.ndarray] = None      # initial physical state (the request payload)
+    Q0: Optional[np.ndarray] = None      # initial physical state (the payload); None: built on the device
     admitted: Optional[float] = None
     finished: Optional[float] = None
     result: Optional[np.ndarray] = None
@@ -115,8 +115,10 @@ def serve(requests: List[Request], hbm_budget: int, max_cases: int = 64, use_gra
     if own:
         batch = Batch(hbm_budget, max_cases=max_cases, use_graph=use_graph, autotune=autotune)
     try:
+        # a request without a payload is built on the device at admission
+        # when its case has a device initial condition, on the host otherwise
         for r in requests:
-            if r.Q0 is None:
+            if r.Q0 is None and r.spec.device_ic() is None:
                 r.Q0 = initial_state(r.spec)
         if pretune and autotune:
             seen = set()
@@ -125,7 +127,7 @@ def serve(requests: List[Request], hbm_budget: int, max_cases: int = 64, use_gra
                 key = (s.precision, s.scheme, s.order, s.nx, s.ny, s.nz)
                 if key not in seen:
                     seen.add(key)
-                    batch.tune(s, r.Q0)
+                    batch.tune(s, r.Q0 if r.Q0 is not None else initial_state(s))
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting: collections.deque = collections.deque()
         by_id: Dict[int, Request] = {}

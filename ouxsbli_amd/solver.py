@@ -60,6 +60,14 @@ class Solver:
         Q = self._check_shape(Q)
         _native.check(self._lib.ox_set_state(self._h, _dp(Q)))
 
+    def init_state(self, ic=None) -> None:
+        """Builds the initial state on the device (ox_init_state) from an
+        ``ox_ic``, by default the case's own (CaseSpec.device_ic)."""
+        ic = ic if ic is not None else self.spec.device_ic()
+        if ic is None:
+            raise ValueError(f"no device initial condition for ic {self.spec.ic!r}")
+        _native.check(self._lib.ox_init_state(self._h, ctypes.byref(ic)))
+
     def get_state(self) -> np.ndarray:
         Q = np.empty(self.spec.state_shape())
         _native.check(self._lib.ox_get_state(self._h, _dp(Q)))

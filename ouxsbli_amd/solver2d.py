@@ -55,6 +55,11 @@ class Solver2D:
             raise ValueError(f"state shape {Q.shape} != {self.spec.state_shape()}")
         _native.check(self._lib.ox2d_set_state(self._h, _dp(Q)))
 
+    def init_state(self) -> None:
+        """The oblique-shock initial state of the case, built on the device
+        (ox2d_init_state)."""
+        _native.check(self._lib.ox2d_init_state(self._h))
+
     def get_state(self) -> np.ndarray:
         Q = np.empty(self.spec.state_shape(), dtype=np.float64)
         _native.check(self._lib.ox2d_get_state(self._h, _dp(Q)))

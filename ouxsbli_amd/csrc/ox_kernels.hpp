@@ -825,15 +825,22 @@ __device__ __forceinline__ void sweep_line_tile(const SweepArgs<R, S>& a, int bx
   }
   R Fp[5];
   CellState<DIR, EP> cs;
+  // The prefetches below are issued on every iteration, the last one included
+  // (its values are never used): a prefetch behind a branch makes the
+  // compiler wait for every outstanding memory operation, the residual stores
+  // of the iteration included, at the loop's merge points.  On the last
+  // iteration the cell entering the window is clamped to the array's last
+  // layer along DIR, and cell f+1 = s_end is at most that layer as well.
+  const int n_along = DIR == 1 ? g.ny : g.nz;
   for (int f = s0 - 1; f < s_end; ++f) {
     finish_cell<R, S, DIR, ORDER, SCHEME, VISC, VO>(g, c, nxt, win[St::SW - 1]);
-    if (f + 1 < s_end) {
-      const int sp = f + 1 + St::HW;
+    {
+      const int sp = min(f + 1 + St::HW, n_along - 1);
       fetch_cell<S, DIR, ORDER, SCHEME, VISC, VO>(P, g, mbase + sp * stride, mbase + wrap(sp) * stride, nxt);
     }
     // residual (and state) of cell f+1, consumed one iteration later
     CellState<DIR, EP> ncs;
-    if (f + 1 < s_end) fetch_state<DIR, EP>(Racc, sa, qbase + (f + 1) * qstride, g.nx, ncs);
+    fetch_state<DIR, EP>(Racc, sa, qbase + (f + 1) * qstride, g.nx, ncs);
     const int q = qbase + f * qstride;  // cell f
     const double *Ra = cs.R, *Qa = cs.A, *Qb = cs.B, *Qc = cs.C;
     R F[5];

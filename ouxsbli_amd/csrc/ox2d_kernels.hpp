@@ -124,19 +124,21 @@ __device__ __forceinline__ void flux2(const PhysC<R>& c, int variant, const R* L
     F[2] = F5[2];
     F[3] = F5[4];
   } else {
-    R o[5];
+    // roe_local2: the 3-D routine's values with the zero component's terms
+    // removed (normal u, tangential v for x faces; normal v, tangential u for y)
+    R o[4];
     if constexpr (DIR == 0) {
-      ox::roe_local<R>(c, L[0], Rr[0], L[1], Rr[1], L[2], Rr[2], R(0), R(0), L[3], Rr[3], o);
+      ox::roe_local2<R>(c, L[0], Rr[0], L[1], Rr[1], L[2], Rr[2], L[3], Rr[3], o);
       F[0] = o[0];
       F[1] = o[1];
       F[2] = o[2];
-      F[3] = o[4];
+      F[3] = o[3];
     } else {
-      ox::roe_local<R>(c, L[0], Rr[0], L[2], Rr[2], R(0), R(0), L[1], Rr[1], L[3], Rr[3], o);
+      ox::roe_local2<R>(c, L[0], Rr[0], L[2], Rr[2], L[1], Rr[1], L[3], Rr[3], o);
       F[0] = o[0];
-      F[1] = o[3];
+      F[1] = o[2];
       F[2] = o[1];
-      F[3] = o[4];
+      F[3] = o[3];
     }
   }
 }

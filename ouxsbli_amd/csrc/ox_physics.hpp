@@ -378,6 +378,58 @@ OX_HD void roe_local(const PhysC<R>& c, R rho1, R rho2, R u1, R u2, R v1, R v2, 
   F[4] = F[4] - R(0.5) * (a1 * (H - cu) * R1 + a2 * (-(w * cr * R2) + v * cr * R3 + q2 * R4) + a3 * (H + cu) * R5);
 }
 
+// roe_local for a face with one tangential velocity (the 2-D solver, where
+// the third component is zero): the same operations with the terms that
+// multiply or add that exact zero removed.  Removing them changes no value
+// (x + 0 and x - 0 are x; only the sign of a zero result can differ), and
+// the tangential component may sit in either tangential slot of roe_local:
+// with it in the w slot, R2 there is -R3 here exactly, so both face
+// directions of the 2-D solver reduce to this one routine.
+// o[4] = (mass, normal momentum, tangential momentum, energy).
+template <class R>
+OX_HD void roe_local2(const PhysC<R>& c, R rho1, R rho2, R un1, R un2, R ut1, R ut2, R p1, R p2, R* o) {
+  const R k1 = R(0.5) * (un1 * un1 + ut1 * ut1);
+  const R k2 = R(0.5) * (un2 * un2 + ut2 * ut2);
+  const R e1 = p1 * c.over_gamma_1 + rho1 * k1;
+  const R e2 = p2 * c.over_gamma_1 + rho2 * k2;
+  const R rhou1 = rho1 * un1;
+  const R rhou2 = rho2 * un2;
+  o[0] = R(0.5) * (rhou1 + rhou2);
+  o[1] = R(0.5) * (rhou1 * un1 + rhou2 * un2 + (p1 + p2));
+  o[2] = R(0.5) * (rhou1 * ut1 + rhou2 * ut2);
+  o[3] = R(0.5) * ((e1 + p1) * un1 + (e2 + p2) * un2);
+  const R dQ0 = -rho1 + rho2;
+  const R dQ1 = -(rho1 * un1) + rho2 * un2;
+  const R dQt = -(rho1 * ut1) + rho2 * ut2;
+  const R dQ4 = -e1 + e2;
+  const R H1 = c.gamma_over_gamma_1 * p1 / rho1 + k1;
+  const R H2 = c.gamma_over_gamma_1 * p2 / rho2 + k2;
+  const R rhol = fsqrt(rho1);
+  const R rhor = fsqrt(rho2);
+  const R rr = R(1) / (rhol + rhor);
+  const R u = (rhol * un1 + rhor * un2) * rr;
+  const R t = (rhol * ut1 + rhor * ut2) * rr;
+  const R H = (rhol * H1 + rhor * H2) * rr;
+  const R cr = fsqrt(c.gamma_1 * (H - R(0.5) * (u * u + t * t)));
+  const R q2 = R(0.5) * (u * u + t * t);
+  const R oc = R(1) / cr;
+  const R b2 = c.gamma_1 * oc * oc;
+  const R b1 = q2 * b2;
+  const R R1 = R(0.5) * ((b1 + u * oc) * dQ0 - (oc + b2 * u) * dQ1 - b2 * t * dQt + b2 * dQ4);
+  const R R3 = oc * (-(t * dQ0) + dQt);
+  const R R4 = (R(1) - b1) * dQ0 + b2 * (u * dQ1 + t * dQt - dQ4);
+  const R R5 = R(0.5) * ((b1 - u * oc) * dQ0 + (oc - b2 * u) * dQ1 - b2 * (t * dQt - dQ4));
+  const R a1 = fabs_(u - cr);
+  const R a2 = fabs_(u);
+  const R a3 = fabs_(u + cr);
+  const R a2c = a2 * cr;
+  const R cu = cr * u;
+  o[0] = o[0] - R(0.5) * (a1 * R1 + a2 * R4 + a3 * R5);
+  o[1] = o[1] - R(0.5) * (a1 * (u - cr) * R1 + a2 * u * R4 + a3 * (u + cr) * R5);
+  o[2] = o[2] - R(0.5) * (a1 * t * R1 + a2c * R3 + a2 * t * R4 + a3 * t * R5);
+  o[3] = o[3] - R(0.5) * (a1 * (H - cu) * R1 + a2 * (t * cr * R3 + q2 * R4) + a3 * (H + cu) * R5);
+}
+
 // MUSCL left/right states of the five primitives (delta_r2/4/6 of
 // calc_roe_kernel_internal.f90.fypp:28-55 = the delta4/delta6 of SLAU).
 template <class R, int N>

@@ -375,6 +375,13 @@ int ox_batch_admit_ic(ox_batch* b, const ox_config* cfg, const ox_ic* ic, int32_
  * replacing ox2d_set_state of a host array (2D_solver/OS/set.f90:28-46). */
 int ox2d_init_state(ox2d_solver* s);
 
+/* Page-locked host memory (hipHostMalloc): a device-to-host copy of a result
+ * into it runs at the host link's full rate, where a copy into pageable
+ * memory is staged.  No counterpart in the reference; a serving helper for
+ * callers of ox_batch_retire / ox_get_state that keep result buffers. */
+int ox_host_alloc(uint64_t bytes, void** ptr);
+int ox_host_free(void* ptr);
+
 /* Thread-local text of the last error on this thread ("" if none). */
 const char* ox_last_error(void);
 

@@ -204,6 +204,8 @@ EXPORTS = (
     "ox_init_state",
     "ox_batch_admit_ic",
     "ox2d_init_state",
+    "ox_host_alloc",
+    "ox_host_free",
     "ox_last_error",
     "ox_version",
 )

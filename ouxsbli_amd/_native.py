@@ -104,6 +104,10 @@ def load() -> ctypes.CDLL:
     L.ox_batch_admit_ic.restype = I
     L.ox2d_init_state.argtypes = [P]
     L.ox2d_init_state.restype = I
+    L.ox_host_alloc.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]
+    L.ox_host_alloc.restype = I
+    L.ox_host_free.argtypes = [P]
+    L.ox_host_free.restype = I
     L.ox_last_error.argtypes = []
     L.ox_last_error.restype = ctypes.c_char_p
     L.ox_version.argtypes = []

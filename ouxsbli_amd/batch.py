@@ -35,9 +35,72 @@ def footprint(spec) -> int:
     return int(b.value)
 
 
+class PinnedPool:
+    """Page-locked host buffers (ox_host_alloc) for results, reused by size.
+
+    ``array(shape)`` returns a float64 numpy array on a pinned buffer; when
+    the array (and every view of it) is gone, the buffer goes back to the
+    pool for the next result of the same size, up to ``keep_bytes`` retained
+    in all; beyond that it is freed."""
+
+    def __init__(self, keep_bytes: int = 8 << 30):
+        self._lib = _native.load()
+        self._free = {}
+        self._kept = 0
+        self.keep_bytes = keep_bytes
+
+    def array(self, shape) -> np.ndarray:
+        n = int(np.prod(shape))
+        nbytes = 8 * n
+        stack = self._free.get(nbytes)
+        if stack:
+            ptr = stack.pop()
+            self._kept -= nbytes
+        else:
+            p = ctypes.c_void_p()
+            _native.check(self._lib.ox_host_alloc(nbytes, ctypes.byref(p)))
+            ptr = p.value
+        buf = (ctypes.c_double * n).from_address(ptr)
+        buf._release = _Release(self, ptr, nbytes)  # returns the buffer when the last view is gone
+        return np.frombuffer(buf, dtype=np.float64).reshape(shape)
+
+    def _give_back(self, ptr: int, nbytes: int) -> None:
+        if self._kept + nbytes <= self.keep_bytes:
+            self._free.setdefault(nbytes, []).append(ptr)
+            self._kept += nbytes
+        else:
+            self._lib.ox_host_free(ctypes.c_void_p(ptr))
+
+
+class _Release:
+    __slots__ = ("pool", "ptr", "nbytes")
+
+    def __init__(self, pool, ptr, nbytes):
+        self.pool, self.ptr, self.nbytes = pool, ptr, nbytes
+
+    def __del__(self):
+        try:
+            self.pool._give_back(self.ptr, self.nbytes)
+        except Exception:  # interpreter shutdown: the process frees its pinned memory
+            pass
+
+
+_POOL: Optional[PinnedPool] = None
+
+
+def pinned_pool() -> PinnedPool:
+    """The process's pool of pinned result buffers."""
+    global _POOL
+    if _POOL is None:
+        _POOL = PinnedPool()
+    return _POOL
+
+
 class Batch:
     def __init__(self, hbm_budget: int, max_cases: int = 64, use_graph: bool = True, autotune: bool = True,
-                 device: int = 0):
+                 device: int = 0, pinned_results: bool = True):
+        """``pinned_results``: retire() returns arrays on page-locked host
+        buffers from the process's pool (a faster device-to-host copy)."""
         self._lib = _native.load()
         c = _abi.OxBatchConfig()
         c.abi_version = _abi.OX_ABI_VERSION
@@ -50,6 +113,7 @@ class Batch:
         _native.check(self._lib.ox_batch_create(ctypes.byref(c), ctypes.byref(h)))
         self._h = h
         self._shapes = {}
+        self._pinned = pinned_results
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
@@ -132,7 +196,8 @@ class Batch:
 
     def retire(self, cid: int) -> np.ndarray:
         """Final physical state of a finished case (ghost layers filled); frees it."""
-        Q = np.empty(self._shapes[cid], dtype=np.float64)
+        shape = self._shapes[cid]
+        Q = pinned_pool().array(shape) if self._pinned else np.empty(shape, dtype=np.float64)
         _native.check(self._lib.ox_batch_retire(self._h, int(cid), Q.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         del self._shapes[cid]
         return Q

@@ -225,6 +225,19 @@ extern "C" {
 
 const char* ox_last_error(void) { return g_err.c_str(); }
 
+int ox_host_alloc(uint64_t bytes, void** ptr) {
+  if (!ptr || bytes == 0) return ox_fail(OX_EINVAL, "null pointer or zero size");
+  *ptr = nullptr;
+  OX_HIP(hipHostMalloc(ptr, (size_t)bytes, hipHostMallocDefault));
+  return OX_OK;
+}
+
+int ox_host_free(void* ptr) {
+  if (!ptr) return OX_OK;
+  OX_HIP(hipHostFree(ptr));
+  return OX_OK;
+}
+
 const char* ox_version(void) { return "ouxsbli-mi355x 1 gfx950"; }
 
 int ox_create(const ox_config* cfg, ox_solver** out) {

@@ -68,9 +68,9 @@ def kernel_model(spec, ops):
 MEASURED_TRAFFIC_256 = {
     "prim": 1957e6,
     "sweep_x": 2282e6,
-    "sweep_y": 2671e6,
-    "sweep_z_rk": (4746e6 + 2 * 5524e6) / 3,
-    "ghost_fill": 175e6,
+    "sweep_y": 2653e6,
+    "sweep_z_rk": (4667e6 + 2 * 5435e6) / 3,
+    "ghost_fill": 176e6,
 }
 
 

@@ -184,15 +184,23 @@ __device__ __forceinline__ void face2(const Args2<R>& a, int f, int n, const R (
 }
 
 // Conserved state of one point of a 2-D case (Fortran order (nx,4,ny)).
+// The state pointers come from a descriptor in memory, so the compiler cannot
+// tell that they point to global memory and would use flat loads and stores,
+// which also count on the LDS wait counter: every LDS wait of the tile would
+// then wait for the outstanding state stores too.  They are cast to the global
+// address space here.
+using gdouble = __attribute__((address_space(1))) double;
 __device__ __forceinline__ void load_q(const double* __restrict__ Q, int nx, int ny, int i, int j, double* q) {
+  const gdouble* G = (const gdouble*)Q;
   const size_t b = (size_t)i + (size_t)nx * 4 * (size_t)j;
 #pragma unroll
-  for (int l = 0; l < 4; ++l) q[l] = Q[b + (size_t)l * nx];
+  for (int l = 0; l < 4; ++l) q[l] = G[b + (size_t)l * nx];
 }
 __device__ __forceinline__ void store_q(double* __restrict__ Q, int nx, int i, int j, const double* q) {
+  gdouble* G = (gdouble*)Q;
   const size_t b = (size_t)i + (size_t)nx * 4 * (size_t)j;
 #pragma unroll
-  for (int l = 0; l < 4; ++l) Q[b + (size_t)l * nx] = q[l];
+  for (int l = 0; l < 4; ++l) G[b + (size_t)l * nx] = q[l];
 }
 
 // One tile: interior cells x0..x0+TX-1, y0..y0+TY-1 (0-based point indices;

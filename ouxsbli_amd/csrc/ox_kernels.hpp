@@ -28,6 +28,16 @@ template <class S> struct Prim {
   S *rho, *u, *v, *w, *p, *T, *mu, *s, *mut, *qc2;
 };
 
+// The pointer seen as one into global memory.  Pointers a kernel receives as
+// arguments are known global; those the batched kernels read from their
+// descriptor arrays are not, and accesses through them would compile to flat
+// loads and stores (addresses in VGPRs only, and counted on the LDS wait
+// counter as well).  Every access to primitives, residual and state goes
+// through G.
+template <class T> __device__ __forceinline__ __attribute__((address_space(1))) T* G(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 constexpr int KEEP = OX_SCHEME_KEEP;
 constexpr int SLAU = OX_SCHEME_SLAU;
 constexpr int ROE = OX_SCHEME_ROE;
@@ -84,13 +94,13 @@ __device__ __forceinline__ PrimVals prim_of(double q1, double q2, double q3, dou
 }
 template <class S, bool VISC>
 __device__ __forceinline__ void store_prim(const Prim<S>& P, int m, const PrimVals& r) {
-  P.rho[m] = (S)r.rho;
-  P.u[m] = (S)r.u;
-  P.v[m] = (S)r.v;
-  P.w[m] = (S)r.w;
-  P.p[m] = (S)r.p;
-  P.T[m] = (S)r.T;
-  if constexpr (VISC) P.mu[m] = (S)r.mu;
+  G(P.rho)[m] = (S)r.rho;
+  G(P.u)[m] = (S)r.u;
+  G(P.v)[m] = (S)r.v;
+  G(P.w)[m] = (S)r.w;
+  G(P.p)[m] = (S)r.p;
+  G(P.T)[m] = (S)r.T;
+  if constexpr (VISC) G(P.mu)[m] = (S)r.mu;
 }
 
 template <class S, bool VISC>
@@ -184,8 +194,8 @@ __global__ __launch_bounds__(256) void k_les_mut(Prim<S> P, Geom g, PhysC<R> c) 
                ((gr[3] + gr[1]) * (gr[3] + gr[1]) + (gr[7] + gr[5]) * (gr[7] + gr[5]) +
                 (gr[2] + gr[6]) * (gr[2] + gr[6]));
   const R nut = ftheta * R(0.06) * fsqrt(fsqrt(S2)) * fsqrt(fsqrt(q2)) * c.les_delta15;
-  P.qc2[m] = (S)q2;
-  P.mut[m] = (S)(R(P.rho[m]) * nut);
+  G(P.qc2)[m] = (S)q2;
+  G(P.mut)[m] = (S)(R(G(P.rho)[m]) * nut);
 }
 
 // set_bc_mut_common (3D_solver/src/set_bc_common.f90:296-335): pass 0 = x
@@ -266,18 +276,18 @@ __device__ __forceinline__ void cross_fields(const Prim<S>& P, const Geom& g, co
 // Issues the loads of cell m (sensor read at ms, its periodic image).
 template <class S, int DIR, int ORDER, int SCHEME, int VISC, int VO, bool LOAD_S = true>
 __device__ __forceinline__ void fetch_cell(const Prim<S>& P, const Geom& g, int m, int ms, RawCell<S, VISC, VO>& r) {
-  r.rho = P.rho[m];
-  r.u = P.u[m];
-  r.v = P.v[m];
-  r.w = P.w[m];
-  r.p = P.p[m];
-  if constexpr (kUsesT<SCHEME> || VISC) r.T = P.T[m];
-  if constexpr (LOAD_S && kUsesSensor<SCHEME, ORDER>) r.s = P.s[ms];
+  r.rho = G(P.rho)[m];
+  r.u = G(P.u)[m];
+  r.v = G(P.v)[m];
+  r.w = G(P.w)[m];
+  r.p = G(P.p)[m];
+  if constexpr (kUsesT<SCHEME> || VISC) r.T = G(P.T)[m];
+  if constexpr (LOAD_S && kUsesSensor<SCHEME, ORDER>) r.s = G(P.s)[ms];
   if constexpr (VISC) {
-    r.mu = P.mu[m];
+    r.mu = G(P.mu)[m];
     if constexpr (VISC == 2) {
-      r.mt = P.mut[m];
-      r.qc2 = P.qc2[m];
+      r.mt = G(P.mut)[m];
+      r.qc2 = G(P.qc2)[m];
     }
     const S* f[4];
     int st[4];
@@ -288,7 +298,7 @@ __device__ __forceinline__ void fetch_cell(const Prim<S>& P, const Geom& g, int 
 #pragma unroll
       for (int t = 0; t < VC; ++t) {
         const int off = t < VC / 2 ? t - VC / 2 : t - VC / 2 + 1;
-        r.x[q][t] = f[q][m + off * st[q]];
+        r.x[q][t] = G(f[q])[m + off * st[q]];
       }
     }
   }
@@ -619,13 +629,13 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<R, S>& a, int bx, i
           uzm = R(r.x[2][H - 1]); uzp = R(r.x[2][H]);
           wzm = R(r.x[3][H - 1]); wzp = R(r.x[3][H]);
         } else {
-          uym = R(P.u[q - sy]); uyp = R(P.u[q + sy]);
-          vym = R(P.v[q - sy]); vyp = R(P.v[q + sy]);
-          uzm = R(P.u[q - sz]); uzp = R(P.u[q + sz]);
-          wzm = R(P.w[q - sz]); wzp = R(P.w[q + sz]);
+          uym = R(G(P.u)[q - sy]); uyp = R(G(P.u)[q + sy]);
+          vym = R(G(P.v)[q - sy]); vyp = R(G(P.v)[q + sy]);
+          uzm = R(G(P.u)[q - sz]); uzp = R(G(P.u)[q + sz]);
+          wzm = R(G(P.w)[q - sz]); wzp = R(G(P.w)[q + sz]);
         }
-        dp = ducros_yz<R>(g, uym, uyp, vym, vyp, R(P.w[q - sy]), R(P.w[q + sy]), uzm, uzp, R(P.v[q - sz]),
-                          R(P.v[q + sz]), wzm, wzp);
+        dp = ducros_yz<R>(g, uym, uyp, vym, vyp, R(G(P.w)[q - sy]), R(G(P.w)[q + sy]), uzm, uzp, R(G(P.v)[q - sz]),
+                          R(G(P.v)[q + sz]), wzm, wzp);
       }
     }
     CD d;
@@ -646,18 +656,18 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<R, S>& a, int bx, i
         const R fd = ducros_x<R>(g, dp, tile[1][ty][m - 1], tile[1][ty][m + 1], tile[2][ty][m - 1],
                                  tile[2][ty][m + 1], tile[3][ty][m - 1], tile[3][ty][m + 1]);
         tile[7][ty][m] = fd;
-        if (i >= i0 && i < i0 + XT && i < g.h + g.ix) P.s[cidx(g, i, j, k)] = (S)fd;  // one tile stores a cell
+        if (i >= i0 && i < i0 + XT && i < g.h + g.ix) G(P.s)[cidx(g, i, j, k)] = (S)fd;  // one tile stores a cell
       }
     } else {
       if (rowok && m >= St::HW - 1 && m <= St::HW + XT && i <= g.nx - 1) {
         const int ic = min(max(i, 1), g.nx - 2);
         const int q = cidx(g, ic, j, k);
-        const R fd = ducros<R>(g, R(P.u[q - 1]), R(P.u[q + 1]), R(P.v[q - 1]), R(P.v[q + 1]), R(P.w[q - 1]),
-                               R(P.w[q + 1]), R(P.u[q - sy]), R(P.u[q + sy]), R(P.v[q - sy]), R(P.v[q + sy]),
-                               R(P.w[q - sy]), R(P.w[q + sy]), R(P.u[q - sz]), R(P.u[q + sz]), R(P.v[q - sz]),
-                               R(P.v[q + sz]), R(P.w[q - sz]), R(P.w[q + sz]));
+        const R fd = ducros<R>(g, R(G(P.u)[q - 1]), R(G(P.u)[q + 1]), R(G(P.v)[q - 1]), R(G(P.v)[q + 1]), R(G(P.w)[q - 1]),
+                               R(G(P.w)[q + 1]), R(G(P.u)[q - sy]), R(G(P.u)[q + sy]), R(G(P.v)[q - sy]), R(G(P.v)[q + sy]),
+                               R(G(P.w)[q - sy]), R(G(P.w)[q + sy]), R(G(P.u)[q - sz]), R(G(P.u)[q + sz]), R(G(P.v)[q - sz]),
+                               R(G(P.v)[q + sz]), R(G(P.w)[q - sz]), R(G(P.w)[q + sz]));
         tile[7][ty][m] = fd;
-        if (i >= i0 && i < i0 + XT && i < g.h + g.ix) P.s[q] = (S)fd;
+        if (i >= i0 && i < i0 + XT && i < g.h + g.ix) G(P.s)[q] = (S)fd;
       }
     }
     __syncthreads();
@@ -706,7 +716,7 @@ __device__ __forceinline__ void sweep_x_tile(const SweepArgs<R, S>& a, int bx, i
   const int ic = i0 + tx;
   if (tx < XT && ic < g.h + g.ix) {
 #pragma unroll
-    for (int l = 0; l < 5; ++l) Racc[qidx(g, ic, l, j, k)] = (S)(cx * (-(double)F[l] + Fr[l]));
+    for (int l = 0; l < 5; ++l) G(Racc)[qidx(g, ic, l, j, k)] = (S)(cx * (-(double)F[l] + Fr[l]));
   }
 }
 
@@ -748,19 +758,19 @@ template <int DIR, int EP, class S>
 __device__ __forceinline__ void fetch_state(const S* __restrict__ Racc, const StageArgs& sa, int q, int st,
                                             CellState<DIR, EP>& cs) {
 #pragma unroll
-  for (int l = 0; l < 5; ++l) cs.R[l] = (double)Racc[q + l * st];
+  for (int l = 0; l < 5; ++l) cs.R[l] = (double)G(Racc)[q + l * st];
   if constexpr (DIR == 2) {
     if constexpr (EP == EP_RK3_S1 || EP == EP_RK3_S23 || EP == EP_RK4_S) {
 #pragma unroll
-      for (int l = 0; l < 5; ++l) cs.A[l] = sa.Qin[q + l * st];
+      for (int l = 0; l < 5; ++l) cs.A[l] = G(sa.Qin)[q + l * st];
     }
     if constexpr (EP == EP_RK3_S23 || EP == EP_RK4_S4) {
 #pragma unroll
-      for (int l = 0; l < 5; ++l) cs.B[l] = sa.Qout[q + l * st];
+      for (int l = 0; l < 5; ++l) cs.B[l] = G(sa.Qout)[q + l * st];
     }
     if constexpr (EP == EP_RK4_S || EP == EP_RK4_S4) {
 #pragma unroll
-      for (int l = 0; l < 5; ++l) cs.C[l] = sa.Rs[q + l * st];
+      for (int l = 0; l < 5; ++l) cs.C[l] = G(sa.Rs)[q + l * st];
     }
   }
 }
@@ -851,7 +861,7 @@ __device__ __forceinline__ void sweep_line_tile(const SweepArgs<R, S>& a, int bx
       for (int l = 0; l < 5; ++l) Rl[l] = Ra[l] + cd * (-(double)Fp[l] + (double)F[l]);
       if constexpr (DIR == 1) {
 #pragma unroll
-        for (int l = 0; l < 5; ++l) Racc[q + l * g.nx] = (S)Rl[l];
+        for (int l = 0; l < 5; ++l) G(Racc)[q + l * g.nx] = (S)Rl[l];
       } else {
         // stage epilogue (3D_solver/src/calc_steps.f90:60-69, :98-106, :137-145, :173-183)
         double Qn[5];
@@ -865,21 +875,21 @@ __device__ __forceinline__ void sweep_line_tile(const SweepArgs<R, S>& a, int bx
 #pragma unroll
           for (int l = 0; l < 5; ++l) {
             Qn[l] = Qa[l] - sa.c1 * Rl[l];
-            sa.Rs[q + l * g.nx] = Qc[l] + sa.c2 * Rl[l];
+            G(sa.Rs)[q + l * g.nx] = Qc[l] + sa.c2 * Rl[l];
           }
         } else if constexpr (EP == EP_RK4_S4) {
 #pragma unroll
           for (int l = 0; l < 5; ++l) {
             const double r = Qc[l] + Rl[l];
             Qn[l] = Qb[l] - r * sa.c1;  // c1 = one_sixth
-            sa.Rs[q + l * g.nx] = 0.0;
+            G(sa.Rs)[q + l * g.nx] = 0.0;
           }
         } else {
 #pragma unroll
           for (int l = 0; l < 5; ++l) Qn[l] = Rl[l];
         }
 #pragma unroll
-        for (int l = 0; l < 5; ++l) sa.Qout[q + l * g.nx] = Qn[l];
+        for (int l = 0; l < 5; ++l) G(sa.Qout)[q + l * g.nx] = Qn[l];
         if constexpr (EP != EP_RHS) {
           // primitives of the new state for the next stage (what k_prim
           // would compute from it), into the other primitive set, at the

@@ -327,8 +327,8 @@ def main():
                     help="requests of the Kelvin-Helmholtz serving stream (0: skip the serving leg)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no multi-GPU path; "
-              "the z-slab sharding design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the library holds each case on one GPU "
+              "and has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
     if args.os_only:

@@ -1,8 +1,8 @@
 // ox_oracle.hpp -- FP64 CPU transcription of the reference RHS + Runge-Kutta
 // path (the ORACLE).
 //
-// TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP library is
-// compared against.  Only tests/, __graft_entry__.smoke() and bench.py's
+// TEST INFRASTRUCTURE ONLY.  This file is the CPU reference that the HIP
+// library is compared against.  Only tests/, __graft_entry__.smoke() and bench.py's
 // cpu_baseline leg may build, link or call it; the product never does.
 //
 // It is a statement-by-statement restatement of the reference's CUDA Fortran

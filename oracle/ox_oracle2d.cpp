@@ -25,7 +25,7 @@
 // Roe.  The 2-D solver has no Roe scheme (2D_solver/src holds KEEP, SLAU and
 // Hybrid only), and the 3-D Roe boundary kernel compares io with 4 and 6 and
 // is therefore first order everywhere (3D_solver/src/calc_roe_kernel.f90.fypp:43-92).
-// The 2-D Roe path here is this build's definition: the 3-D Roe flux
+// The 2-D Roe path here is defined by this library: the 3-D Roe flux
 // (3D_solver/src/calc_roe_3d.f90:1-88, local frame of
 // calc_roe_kernel_internal.f90.fypp:161-163 for y) with w = 0, behind the
 // same order fallback as the SLAU boundary kernels, without the wiggle flag.

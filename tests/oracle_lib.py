@@ -1,7 +1,7 @@
 """ctypes loader for the ORACLE (oracle/liboxoracle.so).
 
-Test infrastructure: the oracle is the checker the HIP library is compared
-against.  Builds the library on first use if it is missing (make -C oracle).
+Test infrastructure: the oracle is the CPU reference the HIP library is
+compared against.  Builds the library on first use if it is missing (make -C oracle).
 """
 import ctypes
 import os

@@ -192,6 +192,14 @@ int ox_footprint(const ox_config* cfg, uint64_t* bytes);
  * shapes are fixed at build time, calc_flux_base.f90.fypp:44-63). */
 int ox_autotune(ox_solver* s, int32_t* lseg_y, int32_t* lseg_z);
 
+/* Sets the y- and z-sweep segment lengths (cells one thread marches along the
+ * sweep direction) explicitly, as ox_autotune would after timing them: any
+ * value in 1..4096.  Results do not depend on the choice; the setter exists so
+ * that a caller can fix a tuned value, and a test can cover segment seams
+ * deterministically.  No reference counterpart (fixed launch shapes,
+ * calc_flux_base.f90.fypp:44-63). */
+int ox_set_segments(ox_solver* s, int32_t lseg_y, int32_t lseg_z);
+
 /* ---------------------------------------------------------------------------
  * Batch engine: many cases resident on one GPU, advanced together.  Cases
  * that share a kernel specialisation and RK scheme form a group; one RK stage
@@ -233,7 +241,8 @@ void ox_batch_destroy(ox_batch* b);
 /* Admits one case: physical state Q(nx,5,ny,nz) (ghosts included), advanced
  * by nsteps RK steps in later ox_batch_step calls.  Returns OX_EBUSY, with
  * nothing allocated, when the case does not fit in the free slots or the
- * free HBM budget now. */
+ * free HBM budget now, and OX_EINVAL when its footprint exceeds the engine's
+ * whole HBM budget (it could never be admitted). */
 int ox_batch_admit(ox_batch* b, const ox_config* cfg, const double* Q, int32_t nsteps, int64_t* id);
 
 /* Fills the engine's segment-length cache for cases of this configuration

@@ -184,6 +184,7 @@ EXPORTS = (
     "ox_kernel_times",
     "ox_footprint",
     "ox_autotune",
+    "ox_set_segments",
     "ox_batch_create",
     "ox_batch_destroy",
     "ox_batch_admit",

@@ -65,6 +65,8 @@ def load() -> ctypes.CDLL:
     L.ox_footprint.restype = I
     L.ox_autotune.argtypes = [P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
     L.ox_autotune.restype = I
+    L.ox_set_segments.argtypes = [P, ctypes.c_int32, ctypes.c_int32]
+    L.ox_set_segments.restype = I
     L.ox_batch_create.argtypes = [ctypes.POINTER(_abi.OxBatchConfig), ctypes.POINTER(P)]
     L.ox_batch_create.restype = I
     L.ox_batch_destroy.argtypes = [P]

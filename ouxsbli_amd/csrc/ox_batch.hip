@@ -292,6 +292,15 @@ int build_plan(ox_batch* b) {
   std::vector<std::pair<int, const double*>> saved;
   for (auto& up : b->active)
     if (up->s) saved.push_back({up->s->C.cur, up->s->C.prim_for});
+  auto restore = [&]() {
+    size_t k = 0;
+    for (auto& up : b->active)
+      if (up->s) {
+        up->s->C.cur = saved[k].first;
+        up->s->C.prim_for = saved[k].second;
+        ++k;
+      }
+  };
   for (int set = 0; set < 2; ++set) {
     for (size_t li = 0; li < b->launches.size(); ++li) {
       BGroup& G = b->groups[b->launch_grp[li]];
@@ -306,7 +315,11 @@ int build_plan(ox_batch* b) {
         ox::StageArgs sa;
         double coef;
         rk_stage(s, st, &Qsrc, &sa, &coef);
-        if (s->C.prim_for != Qsrc) return ox_fail(OX_ESTATE, "batched case without current primitives");
+        if (s->C.prim_for != Qsrc) {
+          restore();
+          invalidate(b);
+          return ox_fail(OX_ESTATE, "batched case without current primitives");
+        }
         alignas(16) char tmp[3 * kMaxArgs];
         G.ops.make_args(s->C, sa, coef * g.a_yz, coef * g.a_zx, coef * g.a_xy, tmp);
         for (int d = 0; d < 3; ++d) std::memcpy(base + ((size_t)d * n + m) * ab, tmp + (size_t)d * ab, ab);
@@ -315,15 +328,7 @@ int build_plan(ox_batch* b) {
       }
     }
   }
-  {
-    size_t k = 0;
-    for (auto& up : b->active)
-      if (up->s) {
-        up->s->C.cur = saved[k].first;
-        up->s->C.prim_for = saved[k].second;
-        ++k;
-      }
-  }
+  restore();
   OX_HIP(hipMemcpyAsync(b->d_args, b->h_args, 2 * b->args_bytes, hipMemcpyHostToDevice, b->stream));
   OX_HIP(hipStreamSynchronize(b->stream));
   b->phase = 0;
@@ -403,6 +408,9 @@ int admit3d(ox_batch* b, const ox_config* cfg, Init init, int32_t nsteps, int64_
   if (cfg->device != b->cfg.device) return ox_fail(OX_EINVAL, "case device differs from the batch device");
   if (cfg->visc == OX_VISC_LES) return ox_fail(OX_EUNSUPPORTED, "LES cases are not batched (use ox_step)");
   const size_t need = solver_footprint(*cfg);
+  if (need > b->cfg.hbm_budget)
+    return ox_fail(OX_EINVAL, "case footprint " + std::to_string(need) + " B exceeds the whole HBM budget " +
+                                  std::to_string(b->cfg.hbm_budget) + " B: it can never be admitted");
   if ((int)(b->active.size() + b->done.size()) >= b->cfg.max_cases) return ox_fail(OX_EBUSY, "no free slot");
   if (b->bytes_in_use + need > b->cfg.hbm_budget) return ox_fail(OX_EBUSY, "footprint exceeds the free HBM budget");
   ox_solver* s = nullptr;
@@ -503,6 +511,15 @@ int ox_autotune(ox_solver* s, int32_t* lseg_y, int32_t* lseg_z) {
   return OX_OK;
 }
 
+int ox_set_segments(ox_solver* s, int32_t lseg_y, int32_t lseg_z) {
+  if (!s) return ox_fail(OX_EINVAL, "null handle");
+  if (lseg_y < 1 || lseg_y > 4096 || lseg_z < 1 || lseg_z > 4096)
+    return ox_fail(OX_EINVAL, "segment lengths must be in 1..4096");
+  s->C.lseg[0] = lseg_y;
+  s->C.lseg[1] = lseg_z;
+  return OX_OK;
+}
+
 int ox_batch_create(const ox_batch_config* cfg, ox_batch** out) {
   if (!cfg || !out) return ox_fail(OX_EINVAL, "null argument");
   *out = nullptr;
@@ -562,6 +579,9 @@ int ox_batch_admit2d(ox_batch* b, const ox2d_config* cfg, const double* Q, int32
   if (nsteps < 1) return ox_fail(OX_EINVAL, "nsteps must be >= 1");
   if (cfg->device != b->cfg.device) return ox_fail(OX_EINVAL, "case device differs from the batch device");
   const size_t need = solver2d_footprint(*cfg);
+  if (need > b->cfg.hbm_budget)
+    return ox_fail(OX_EINVAL, "case footprint " + std::to_string(need) + " B exceeds the whole HBM budget " +
+                                  std::to_string(b->cfg.hbm_budget) + " B: it can never be admitted");
   if ((int)(b->active.size() + b->done.size()) >= b->cfg.max_cases) return ox_fail(OX_EBUSY, "no free slot");
   if (b->bytes_in_use + need > b->cfg.hbm_budget) return ox_fail(OX_EBUSY, "footprint exceeds the free HBM budget");
   ox2d_solver* s = nullptr;

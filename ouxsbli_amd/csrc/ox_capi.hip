@@ -460,7 +460,7 @@ int ox_diagnostics(ox_solver* s, ox_diag* out) {
       sum[q] = t;
       comp[q] += p[2 * q + 1];
     }
-    wmax = std::max(wmax, p[2 * ox::DIAG_NQ]);
+    wmax = ox::nan_max(wmax, p[2 * ox::DIAG_NQ]);
   }
   const double ncell = (double)g.ix * (double)g.iy * (double)g.iz;
   out->ke = (sum[0] + comp[0]) / ncell;

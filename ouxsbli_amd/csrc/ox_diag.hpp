@@ -30,6 +30,11 @@ __device__ __forceinline__ Comp comp_merge(Comp a, Comp b) {
   return r;
 }
 
+// max that propagates NaN (fmax drops it): a state with a NaN or negative
+// pressure cell reports a NaN wave speed instead of a finite one a caller
+// would size dt from.  Once either operand is NaN the result stays NaN.
+__host__ __device__ __forceinline__ double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+
 constexpr int DIAG_BLOCK = 256;
 constexpr int DIAG_NQ = 5;  // ke, entropy, enstrophy, mass, energy (sums) + max wave speed separately
 
@@ -66,7 +71,7 @@ __global__ __launch_bounds__(DIAG_BLOCK) void k_diag(const double* __restrict__ 
     comp_add(acc[4], Q(i, 4, j, k));
     const double cs = sqrt(gamma * p / rho);
     const double um = fmax(fabs(u), fmax(fabs(v), fabs(w))) + cs;
-    wmax = fmax(wmax, um);
+    wmax = nan_max(wmax, um);
   }
   const int tid = threadIdx.x;
   for (int q = 0; q < DIAG_NQ; ++q) {
@@ -82,7 +87,7 @@ __global__ __launch_bounds__(DIAG_BLOCK) void k_diag(const double* __restrict__ 
         ss[q][tid] = m.s;
         sc[q][tid] = m.c;
       }
-      sm[tid] = fmax(sm[tid], sm[tid + off]);
+      sm[tid] = nan_max(sm[tid], sm[tid + off]);
     }
     __syncthreads();
   }

@@ -25,6 +25,7 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
+from ._native import OxError
 from .batch import Batch, Busy
 from .cases import CaseSpec, initial_state, khi_spec
 
@@ -39,6 +40,7 @@ class Request:
     admitted: Optional[float] = None
     finished: Optional[float] = None
     result: Optional[np.ndarray] = None
+    error: Optional[str] = None          # set when the request was rejected (it can never be admitted)
 
     @property
     def cell_stages(self) -> float:
@@ -79,14 +81,20 @@ class ServeReport:
     steps: int
     host_s: Dict = dataclasses.field(default_factory=dict)  # wall seconds in admit / run / retire calls
 
+    def served(self) -> List[Request]:
+        return [r for r in self.requests if r.error is None]
+
     def latencies(self) -> np.ndarray:
-        return np.array([r.finished - r.arrival for r in self.requests])
+        return np.array([r.finished - r.arrival for r in self.served()])
 
     def summary(self) -> Dict:
         lat = self.latencies()
-        cs = sum(r.cell_stages for r in self.requests)
+        if lat.size == 0:
+            lat = np.array([float("nan")])
+        cs = sum(r.cell_stages for r in self.served())
         return {
             "requests": len(self.requests),
+            "rejected": sum(1 for r in self.requests if r.error is not None),
             "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_latency_s": float(lat.mean()),
@@ -145,9 +153,22 @@ def serve(requests: List[Request], hbm_budget: int, max_cases: int = 64, use_gra
                 ta = clock()
                 try:
                     cid = batch.admit(r.spec, r.Q0, r.nsteps)
-                except Busy:
+                except Busy as e:
                     host["admit"] += clock() - ta
+                    if active == 0:
+                        # nothing resident can free a slot or budget: the
+                        # request does not fit an empty engine
+                        r.error = f"not admissible into an empty engine: {e}"
+                        waiting.popleft()
+                        continue
                     break
+                except OxError as e:
+                    # permanently refused (footprint over the whole budget,
+                    # unsupported configuration): reject, serve the rest
+                    host["admit"] += clock() - ta
+                    r.error = str(e)
+                    waiting.popleft()
+                    continue
                 host["admit"] += clock() - ta
                 r.admitted = clock() - t0
                 by_id[cid] = r

@@ -127,5 +127,9 @@ class Solver:
         _native.check(self._lib.ox_autotune(self._h, ctypes.byref(ly), ctypes.byref(lz)))
         return ly.value, lz.value
 
+    def set_segments(self, lseg_y: int, lseg_z: int) -> None:
+        """Fixes the y/z sweep segment lengths (1..4096 cells)."""
+        _native.check(self._lib.ox_set_segments(self._h, int(lseg_y), int(lseg_z)))
+
     def synchronize(self) -> None:
         _native.check(self._lib.ox_synchronize(self._h))

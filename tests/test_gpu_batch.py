@@ -93,16 +93,21 @@ def test_run_back_to_back_equals_alone(precision, use_graph):
     assert st["steps"] == max(ends)
 
 
-def test_batched_fp64_equals_oracle():
-    spec = cases.tgv_spec(20, 6, Ma=1.25, cfl=0.3, cfl_mode="acoustic", scheme="SLAU", visc="NS", visc_order=6, rk=3,
+@pytest.mark.parametrize("n", [20, 72])
+def test_batched_fp64_equals_oracle(n):
+    """A batched FP64 case equals the oracle bit for bit.  The autotuner is
+    off, so the segment length is the default 64 cells: at n = 72 every line
+    sweep runs two segments (64 + 8) and the x sweep two tiles (58 + 14)."""
+    spec = cases.tgv_spec(n, 6, Ma=1.25, cfl=0.3, cfl_mode="acoustic", scheme="SLAU", visc="NS", visc_order=6, rk=3,
                           tvd="hybrid")
     other = cases.khi_spec(24, 4, "KEEP", "fp64", seed=9)
     Q = cases.initial_state(spec)
+    nsteps = 4 if n <= 32 else 2
     o = Oracle(spec)
     o.set_state(Q)
-    o.step(4)
-    with Batch(hbm_budget=4 * GIB, max_cases=4) as b:
-        a = b.admit(spec, Q, 4)
+    o.step(nsteps)
+    with Batch(hbm_budget=4 * GIB, max_cases=4, autotune=False) as b:
+        a = b.admit(spec, Q, nsteps)
         b.admit(other, cases.initial_state(other), 6)
         while a not in b.done():
             b.step()

@@ -1,0 +1,110 @@
+"""GPU parity across the sweeps' tile and segment seams, at ORDER 6.
+
+The x sweep covers a grid row in tiles of 64 - SW interior cells (58 at
+ORDER 6 with a 6-cell face window), handing the right-face flux of each tile's
+last cell across lanes; the y and z sweeps march segments of ``lseg`` cells
+(64 by default), each segment preloading its stencil window.  The interiors
+below span several of each: 130 x 72 x 136 gives three x tiles (58, 58, 14),
+two y segments (64, 8) and three z segments (64, 64, 8) at the default
+segment length; the DHIT case at 124^3 gives three x tiles and two segments
+per line sweep.  Every result must equal the CPU oracle bit for bit (the FP64
+kernels round every operation as the oracle does), in the index ranges of
+3D_solver/src/calc_slau_kernel_internal.f90.fypp:84-252 (faces io+1..n-io-1
+of every interior line).
+"""
+import numpy as np
+import pytest
+
+from ouxsbli_amd import cases
+from ouxsbli_amd.solver import Solver
+from oracle_lib import Oracle
+from test_gpu_parity import _perturbed_tgv, _spec
+
+pytestmark = pytest.mark.gpu
+
+SEAM_N = (130, 72, 136)
+
+SCHEMES = {
+    # config 2's scheme: SLAU ORDER 6 HR-SLAU2, hybrid TVD, central-6 viscous, RK3
+    "slau6-ns6-rk3": (6, "SLAU", "NS", 6, "hybrid", "HRSLAU2", 3),
+    # the ME4Base viscous stencil with the same convective scheme
+    "slau6-me4-rk3": (6, "SLAU", "NS", 6, "hybrid", "HRSLAU2", 3, "ME4Base"),
+    # config 1's scheme at ORDER 4 (60-cell x tiles)
+    "keep4-ns4-rk4": (4, "KEEP", "NS", 4, "none", "HRSLAU2", 4),
+}
+
+
+def _interior(spec):
+    h = spec.order // 2
+    return (slice(h, spec.nz - h), slice(h, spec.ny - h), slice(None), slice(h, spec.nx - h))
+
+
+def _seam_spec(name):
+    return _spec(*SCHEMES[name], n=SEAM_N, Ma=1.25)
+
+
+@pytest.mark.parametrize("name", list(SCHEMES))
+def test_seam_rhs_and_steps_bitwise(name):
+    spec = _seam_spec(name)
+    Q = _perturbed_tgv(spec, seed=7)
+    o = Oracle(spec)
+    o.set_state(Q)
+    Ro = o.rhs()
+    o.step(2)
+    with Solver(spec) as s:
+        s.set_state(Q)
+        Rg = s.rhs()
+        s.step(2)
+        QJg = s.get_qj()
+    sl = _interior(spec)
+    assert np.abs(Ro[sl]).max() > 0
+    np.testing.assert_array_equal(Rg[sl], Ro[sl])
+    np.testing.assert_array_equal(QJg, o.get_qj())
+
+
+@pytest.mark.parametrize("lseg", [(16, 32), (32, 128), (128, 16), (5, 7), (72, 136)])
+def test_seam_segment_lengths_bitwise(lseg):
+    """Every segment length the autotuner may choose (16..128), odd ones and
+    one segment per line give the oracle's right-hand side."""
+    spec = _seam_spec("slau6-ns6-rk3")
+    Q = _perturbed_tgv(spec, seed=8)
+    o = Oracle(spec)
+    o.set_state(Q)
+    Ro = o.rhs()
+    o.step(1)
+    with Solver(spec) as s:
+        s.set_state(Q)
+        s.set_segments(*lseg)
+        Rg = s.rhs()
+        s.step(1)
+        QJg = s.get_qj()
+    sl = _interior(spec)
+    np.testing.assert_array_equal(Rg[sl], Ro[sl])
+    np.testing.assert_array_equal(QJg, o.get_qj())
+
+
+def test_seam_les_dhit_bitwise():
+    """Config 3's scheme (Hybrid KEEP/SLAU ORDER 6, SMS LES, RK4) on its
+    turbulent field at 124^3: three x tiles and two segments per line sweep."""
+    spec = cases.dhit_spec(124, "fp64", seed=5)
+    Q = cases.initial_state(spec)
+    o = Oracle(spec)
+    o.set_state(Q)
+    Ro = o.rhs()
+    o.step(2)
+    with Solver(spec) as s:
+        s.set_state(Q)
+        Rg = s.rhs()
+        s.step(2)
+        QJg = s.get_qj()
+    sl = _interior(spec)
+    np.testing.assert_array_equal(Rg[sl], Ro[sl])
+    np.testing.assert_array_equal(QJg, o.get_qj())
+
+
+def test_segment_setter_rejects_bad_lengths():
+    spec = _spec(*SCHEMES["keep4-ns4-rk4"], n=(16, 16, 16))
+    with Solver(spec) as s:
+        for bad in ((0, 16), (16, -1), (5000, 16)):
+            with pytest.raises(RuntimeError):
+                s.set_segments(*bad)

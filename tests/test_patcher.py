@@ -141,3 +141,33 @@ class TestCaseBuild:
         assert bc_to_params("periodic") == PeriodicBC().to_params()
         with pytest.raises(ValueError):
             ic_to_params({"type": "nope"})
+
+
+def test_parameters_follow_fortran_integer_semantics():
+    """Integer division truncates toward zero and each value takes its
+    declared type, as a Fortran compiler evaluates the constant expressions
+    of mod_globals.f90; a parameter Case reads that cannot be evaluated
+    raises instead of being skipped."""
+    text = """
+      integer, parameter :: nx = 129
+      integer, parameter :: nh = nx/2, unused_tail = 1
+      real(8), parameter :: third_int = 1/3
+      real(8), parameter :: third = 1.d0/3
+      integer, parameter :: neg = -7/2
+      real(8), parameter :: inv2 = 2**(-1)
+      real(8), parameter :: pi = acos(-1.d0)
+      real(8), parameter :: dx = 2.0d0*pi/(nx-1)
+      real(8), parameter :: small = 1.e-3/4
+      integer, parameter :: trunc = 2.7d0
+    """
+    g = read_parameters(text, required=())
+    assert g["nx"] == 129 and isinstance(g["nx"], int)
+    assert g["third_int"] == 0.0 and isinstance(g["third_int"], float)
+    assert g["third"] == 1.0 / 3.0
+    assert g["neg"] == -3          # toward zero, not floor
+    assert g["inv2"] == 0.0        # 2**(-1) is integer 0 in Fortran
+    assert g["dx"] == 2.0 * g["pi"] / 128
+    assert g["small"] == 1.0e-3 / 4
+    assert g["trunc"] == 2
+    with pytest.raises(ValueError):
+        read_parameters("integer, parameter :: nx = undefined_function(3)\n")

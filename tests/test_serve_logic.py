@@ -157,3 +157,30 @@ def test_runs_back_to_back_between_arrivals():
     assert 0.505 <= r1.admitted <= 0.505 + 2 * 0.01 + 1e-9
     assert r1.finished == pytest.approx(r1.admitted + 0.03)
     assert rep.steps == 100
+
+
+def test_request_that_never_fits_is_rejected():
+    """A head-of-line request whose footprint exceeds the whole budget is
+    refused even by an empty engine; the loop rejects it and serves the
+    rest instead of spinning (engines that report it as Busy and as a
+    permanent error are both handled)."""
+    from ouxsbli_amd._native import OxError
+
+    class PermanentBatch(FakeBatch):
+        def admit(self, spec, Q, nsteps):
+            if self.bytes(spec) > self.budget:
+                raise OxError(-1, "footprint exceeds the whole HBM budget")
+            return super().admit(spec, Q, nsteps)
+
+    for cls in (FakeBatch, PermanentBatch):
+        clock = FakeClock()
+        small, big = khi_spec(8, 4, "KEEP"), khi_spec(24, 4, "KEEP")
+        fb = cls(clock, budget=FakeBatch.bytes(big) - 1, slots=4)
+        reqs = [serve.Request(0, big, 4, 0.0, Q0=np.zeros(1)), serve.Request(1, small, 2, 0.01, Q0=np.zeros(1))]
+        rep = serve.serve(reqs, hbm_budget=0, batch=fb, clock=clock, sleep=clock.sleep, pretune=False)
+        r0, r1 = rep.requests
+        assert r0.error is not None and r0.finished is None
+        assert r1.error is None and r1.finished == pytest.approx(r1.admitted + 0.02)
+        s = rep.summary()
+        assert s["requests"] == 2 and s["rejected"] == 1
+        assert clock.t < 1.0

@@ -61,40 +61,109 @@ def kernel_model(spec, ops):
     return m
 
 
-# HBM bytes per launch of the FP64 kernels at 256^3, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this
-# script, scaled by the calibration a 1 GiB device copy gives (FETCH_SIZE x2, WRITE_SIZE x1):
-# profiles/hbm_traffic_tgv256_slau6_ns6_rk3.md.  Only valid for the kernels as they are built now; remeasure
-# after a kernel change.
-MEASURED_TRAFFIC_256 = {
-    "prim": 1957e6,
-    "sweep_x": 2282e6,
-    "sweep_y": 2653e6,
-    "sweep_z_rk": (4667e6 + 2 * 5435e6) / 3,
-    "ghost_fill": 176e6,
-}
+# Measured HBM bytes per launch of the FP64 kernels at 256^3 (rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE runs of this script, scaled by what the same
+# counters read for a device copy of known size), tagged with the SHA-256 of
+# the kernel sources they were measured on: a kernel change makes them stale,
+# and the bench line then reports traffic null.
+TRAFFIC_FILE = ROOT / "ouxsbli_amd" / "flops" / "traffic.json"
+KERNEL_SOURCES = ("ox_kernels.hpp", "ox_physics.hpp", "ox_stage_impl.hpp", "ox_types.hpp", "ox_core.hpp",
+                  "ox_fp64.hip")
 
 
-def cpu_baseline(spec_small, seconds=15.0):
-    """The FP64 CPU oracle on a bounded sample of the same scheme."""
-    sys.path.insert(0, str(ROOT / "tests"))
+def kernel_sources_sha():
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        h.update((ROOT / "ouxsbli_amd" / "csrc" / name).read_bytes())
+    return h.hexdigest()
+
+
+def measured_traffic(n):
+    """{kernel: HBM bytes per launch} measured on the current kernel sources
+    at this grid, or None."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    t = json.loads(TRAFFIC_FILE.read_text())
+    if t.get("sources_sha256") != kernel_sources_sha() or t.get("n") != n:
+        return None
+    return t["bytes_per_launch"]
+
+
+def host_cpu():
+    """CPU model, the CPUs this process may run on and their physical cores."""
+    model, phys = "unknown", set()
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    try:
+        cur = {}
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines() + [""]:
+            if not line.strip():
+                if cur.get("processor") is not None and int(cur["processor"]) in cpus:
+                    phys.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+                    model = cur.get("model name", model)
+                cur = {}
+                continue
+            k, _, v = line.partition(":")
+            cur[k.strip()] = v.strip()
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": len(cpus), "physical_cores": len(phys) or len(cpus)}
+
+
+def _oracle_rate(spec, nsteps, repeats):
+    """BASELINE.md protocol: 1 warm-up step, then `repeats` timings of
+    `nsteps` steps with a steady clock; the median rate (Mcell-updates/s per
+    RK stage).  Setup and I/O excluded."""
     from ouxsbli_amd import cases
     import oracle_lib
-    o = oracle_lib.Oracle(spec_small)
-    o.set_state(cases.tgv_state(spec_small))
-    o.step(1)  # warm-up
+    o = oracle_lib.Oracle(spec)
+    o.set_state(cases.initial_state(spec))
+    o.step(1)
+    rates = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        o.step(nsteps)
+        rates.append(spec.interior_cells * spec.stages * nsteps / (time.perf_counter() - t0) / 1e6)
+    rates.sort()
+    return rates[len(rates) // 2], rates
+
+
+def cpu_baseline(n_small=64, n_large=128):
+    """The FP64 OpenMP oracle (the parity build) on the host cores, on bounded
+    samples: the headline scheme at 64^3 (10 steps, median of 5) and 128^3
+    (2 steps, median of 3), and configuration 1 in full (64^3 KEEP4 NS RK4,
+    100 steps).  OpenMP threads are bound to cores (OMP_PROC_BIND=close,
+    OMP_PLACES=cores) before the oracle library loads; OMP_NUM_THREADS keeps
+    the value the environment sets, else one thread per physical core."""
+    cpu = host_cpu()
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    os.environ.setdefault("OMP_NUM_THREADS", str(cpu["physical_cores"]))
+    sys.path.insert(0, str(ROOT / "tests"))
+    from ouxsbli_amd import cases
     t0 = time.perf_counter()
-    n = 0
-    while True:
-        o.step(1)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or n >= 500:
-            break
-    rate = spec_small.interior_cells * spec_small.stages * n / el / 1e6
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-    return {"value": rate, "unit": "Mcell-updates/s per RK stage", "cores": cores, "kind": "port",
-            "sample": f"oracle (oracle/liboxoracle.so, FP64 OpenMP) on the same scheme at "
-                      f"{spec_small.interior[0]}^3 interior, 1 warm-up + {n} timed RK3 steps ({el:.1f} s)"}
+    v64, r64 = _oracle_rate(cases.config2_spec(n_small, "fp64"), 10, 5)
+    v128, r128 = _oracle_rate(cases.config2_spec(n_large, "fp64"), 2, 3)
+    # configuration 1 in full
+    import oracle_lib
+    c1 = cases.config1_spec(64)
+    o = oracle_lib.Oracle(c1)
+    o.set_state(cases.initial_state(c1))
+    tc = time.perf_counter()
+    o.step(100)
+    t_c1 = time.perf_counter() - tc
+    threads = int(os.environ["OMP_NUM_THREADS"])
+    return {"value": round(v128, 2), "unit": "Mcell-updates/s per RK stage", "cores": threads, "kind": "port",
+            "sample": f"oracle (oracle/liboxoracle.so, FP64 OpenMP, g++ -O3 -ffp-contract=off) on the headline scheme "
+                      f"(SLAU6 HR-SLAU2 hybrid NS6 RK3) at {n_large}^3: 1 warm-up + 3 x 2 timed steps, median; "
+                      f"{time.perf_counter() - t0:.1f} s in all",
+            "cpu_model": cpu["model"], "physical_cores": cpu["physical_cores"], "logical_cpus": cpu["logical_cpus"],
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
+            "headline_scheme_64": {"value": round(v64, 2), "steps": 10, "repeats": [round(r, 2) for r in r64]},
+            "headline_scheme_128": {"value": round(v128, 2), "steps": 2, "repeats": [round(r, 2) for r in r128]},
+            "config1_full": {"value": round(c1.interior_cells * c1.stages * 100 / t_c1 / 1e6, 2),
+                             "seconds": round(t_c1, 2),
+                             "workload": "NS TGV 64^3 KEEP4 central-4 viscous RK4 Ma 0.1, 100 steps"}}
 
 
 def les_leg(n, steps):
@@ -107,17 +176,18 @@ def les_leg(n, steps):
     from ouxsbli_amd import cases
     spec32 = cases.dhit_spec(n, "fp32_bulk")
     spec64 = cases.dhit_spec(n, "fp64")
-    Q = cases.initial_state(spec32)
     out = {"workload": f"LES decaying isotropic turbulence {n}^3 periodic, E(k)~k^4 exp(-2(k/4)^2), Mt 0.3, "
-                       "Hybrid KEEP/SLAU ORDER 6 (threshold 0.4), SMS SGS, central-6 viscous, RK4",
+                       "Hybrid KEEP/SLAU ORDER 6 (threshold 0.4), SMS SGS, central-6 viscous, RK4; initial field "
+                       "synthesised on the device from its spectral modes",
            "steps": steps}
     res = {}
     for name, spec in (("fp32_bulk", spec32), ("fp64", spec64)):
-        r = run_form_state(spec, Q, steps, 1)
+        r = run_form_state(spec, None, steps, 1)
         res[name] = r
         cs = spec.interior_cells * spec.stages * steps
         out[name] = {"Mcell_updates_per_s_stage": round(cs / r["wall_s"] / 1e6, 1),
-                     "ms_per_step": round(r["wall_s"] * 1e3 / steps, 3)}
+                     "ms_per_step": round(r["wall_s"] * 1e3 / steps, 3),
+                     "initial_state_s": round(r["ic_s"], 3) if r["ic_s"] is not None else None}
     k32, k64 = res["fp32_bulk"]["diag"]["ke"], res["fp64"]["diag"]["ke"]
     out["fp32_bulk"]["ke_rel_err_vs_fp64"] = abs(k32 - k64) / abs(k64)
     out["fp32_bulk"]["speedup_vs_fp64"] = round(out["fp32_bulk"]["Mcell_updates_per_s_stage"] /
@@ -129,7 +199,13 @@ def les_leg(n, steps):
 def run_form_state(spec, Q, steps, warmup):
     from ouxsbli_amd.solver import Solver
     with Solver(spec) as s:
-        s.set_state(Q)
+        if Q is None:
+            t_ic = time.perf_counter()
+            s.init_state()
+            ic_s = time.perf_counter() - t_ic
+        else:
+            s.set_state(Q)
+            ic_s = None
         if warmup:
             s.step(warmup)
         s.synchronize()
@@ -139,7 +215,7 @@ def run_form_state(spec, Q, steps, warmup):
         wall = time.perf_counter() - t0
         diag = s.diagnostics()
         ok, _, _ = s.check_state()
-    return {"wall_s": wall, "diag": diag, "ok": ok}
+    return {"wall_s": wall, "diag": diag, "ok": ok, "ic_s": ic_s}
 
 
 OS_BYTES_PER_CELL_STAGE = (64.0 + 96.0 + 96.0) / 3.0  # 2-D RK3: stage 1 reads Q, writes Q1; stages 2, 3 read two
@@ -244,44 +320,54 @@ SERVE_BUDGET = 64 << 30
 SERVE_SLOTS = 16
 
 
-def serving_leg(n_requests, fp64_value):
-    """Kelvin-Helmholtz serving stream through the batch engine (FP32-bulk):
-    seeded Poisson arrivals at 20 req/s, interior grids 64^3..256^3 (multiples
-    of 16), KEEP or SLAU at ORDER 4 or 6, Euler, RK3, 200 steps each; FIFO
-    admission by HBM footprint (64 GiB budget) and 16 slots, each request's
-    initial state built on the device when it is admitted."""
+def _serve_once(n_requests, precision, seed, batch=None, nsteps=200):
     from ouxsbli_amd import serve
-    # requests carry parameters only: each initial state is built on the device at admission
-    reqs = serve.khi_stream(n_requests, rate_hz=20.0, nsteps=200, precision="fp32_bulk", with_states=False)
-    rep = serve.serve(reqs, hbm_budget=SERVE_BUDGET, max_cases=SERVE_SLOTS, keep_results=False)
+    reqs = serve.khi_stream(n_requests, rate_hz=20.0, nsteps=nsteps, precision=precision, with_states=False,
+                            seed=seed)
+    rep = serve.serve(reqs, hbm_budget=SERVE_BUDGET, max_cases=SERVE_SLOTS, keep_results=False, pretune=False,
+                      batch=batch)
     s = rep.summary()
-    out = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()}
-    out["workload"] = ("3-D Kelvin-Helmholtz stream: double tanh shear layer d=L/28, Ma 0.5, 1e-3 U seeded noise, "
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()}
+
+
+def serving_leg(n_requests, fp64_value):
+    """Kelvin-Helmholtz serving stream through the batch engine: seeded
+    Poisson arrivals at 20 req/s, interior grids 64^3..256^3 (multiples of
+    16), KEEP or SLAU at ORDER 4 or 6, Euler, RK3, 200 steps each; FIFO
+    admission by HBM footprint (64 GiB budget) and 16 slots, each request's
+    initial state built on the device when it is admitted.
+
+    - cold: a fresh engine; segment lengths are tuned when a shape is first
+      admitted, inside that request's latency;
+    - warm: an engine that served an earlier stream (another seed, 2 steps per
+      request) keeps its tuned shapes, as a running server does; shapes it
+      has not seen are tuned on admission as in the cold run;
+    - the same stream in the FP64 plain form on a warm engine, for the
+      FP32-bulk/FP64 ratio on identical requests."""
+    from ouxsbli_amd.batch import Batch
+    seed = 20261207
+    out = {"workload": "3-D Kelvin-Helmholtz stream: double tanh shear layer d=L/28, Ma 0.5, 1e-3 U seeded noise, "
                        f"{n_requests} requests, Poisson 20 req/s, grids 64^3-256^3, KEEP/SLAU ORDER 4/6, Euler RK3, "
-                       "200 steps each, FP32-bulk, initial states built on the device at admission, batched per stage "
-                       "with HIP graph replay")
-    out["hbm_budget_GiB"] = SERVE_BUDGET >> 30
-    out["slots"] = SERVE_SLOTS
-    busy = s["device_busy_Mcell_updates_per_s_stage"]
-    out["device_busy_vs_fp64_plain"] = round(busy / fp64_value, 3) if busy else None
-    # the same requests through the plain FP64 single-case path, one at a time
-    # (1 warm-up step, then a sample of timed steps each, device time)
-    import dataclasses
-    from ouxsbli_amd.solver import Solver
-    sample = 10
-    cs = 0
-    dev_s = 0.0
-    for r in serve.khi_stream(n_requests, rate_hz=20.0, nsteps=200, precision="fp32_bulk", with_states=False):
-        spec = dataclasses.replace(r.spec, precision="fp64")
-        with Solver(spec) as h:
-            h.init_state()
-            h.step(1)
-            dev_s += h.step(sample) * 1e-3
-        cs += spec.interior_cells * spec.stages * sample
-    plain = cs / dev_s / 1e6
-    out["fp64_plain_same_requests"] = {"device_Mcell_updates_per_s_stage": round(plain, 1), "steps_per_request": sample,
-                                       "form": "each request alone through the FP64 single-case path"}
-    out["device_busy_vs_fp64_plain_same_requests"] = round(busy / plain, 3) if busy else None
+                       "200 steps each, FP32-bulk, initial states built on the device at admission, batched per "
+                       "stage with HIP graph replay",
+           "hbm_budget_GiB": SERVE_BUDGET >> 30, "slots": SERVE_SLOTS}
+    cold = _serve_once(n_requests, "fp32_bulk", seed)
+    out.update(cold)
+    out["tuning"] = "cold: each new shape tuned at admission, inside latency"
+    warm = {}
+    for prec in ("fp32_bulk", "fp64"):
+        with Batch(SERVE_BUDGET, max_cases=SERVE_SLOTS, use_graph=True, autotune=True) as b:
+            _serve_once(n_requests, prec, seed + 1000, batch=b, nsteps=2)
+            warm[prec] = _serve_once(n_requests, prec, seed, batch=b)
+    out["warm"] = warm["fp32_bulk"]
+    out["fp64_served_warm"] = warm["fp64"]
+    b32 = warm["fp32_bulk"]["device_busy_Mcell_updates_per_s_stage"]
+    b64 = warm["fp64"]["device_busy_Mcell_updates_per_s_stage"]
+    out["fp32_vs_fp64_same_requests_device_busy"] = round(b32 / b64, 3) if b32 and b64 else None
+    out["fp32_vs_fp64_same_requests_wall"] = round(warm["fp32_bulk"]["Mcell_updates_per_s_stage"] /
+                                                   warm["fp64"]["Mcell_updates_per_s_stage"], 3)
+    busy = cold["device_busy_Mcell_updates_per_s_stage"]
+    out["device_busy_vs_fp64_plain_config2"] = round(busy / fp64_value, 3) if busy else None
     return out
 
 
@@ -305,19 +391,66 @@ def run_form(spec, steps, warmup, profile):
     return {"wall_s": wall, "dev_ms": dev_ms, "kernels": kt, "diag": diag, "ok": ok}
 
 
+def history_leg(n, nsteps, every):
+    """Configuration 2's kinetic-energy and enstrophy histories: the FP64
+    plain and the FP32-bulk forms from the same initial field, diagnostics
+    every `every` steps over `nsteps` steps (the FP64 GPU form stands in for
+    the oracle at this size; it equals the oracle bit for bit on the same
+    scheme at the sizes the oracle runs, tests/test_gpu_seams.py).  Also the
+    sustained rate of each form: device time of all steps, diagnostics
+    excluded."""
+    from ouxsbli_amd import cases
+    from ouxsbli_amd.solver import Solver
+    out = {"workload": f"config 2 ({n}^3 supersonic NS TGV, SLAU6 HR-SLAU2 hybrid NS6 RK3), {nsteps} steps",
+           "every": every}
+    hist = {}
+    for prec in ("fp64", "fp32_bulk"):
+        spec = cases.config2_spec(n, prec)
+        rows = []
+        dev_ms = 0.0
+        t0 = time.perf_counter()
+        with Solver(spec) as s:
+            s.init_state()
+            d = s.diagnostics()
+            rows.append((0, d["ke"], d["enstrophy"]))
+            for k in range(every, nsteps + 1, every):
+                dev_ms += s.step(every)
+                d = s.diagnostics()
+                rows.append((k, d["ke"], d["enstrophy"]))
+            ok, _, _ = s.check_state()
+        wall = time.perf_counter() - t0
+        hist[prec] = rows
+        cs = spec.interior_cells * spec.stages * nsteps
+        out[prec] = {"sustained_Mcell_updates_per_s_stage": round(cs / (dev_ms * 1e-3) / 1e6, 1),
+                     "device_s": round(dev_ms * 1e-3, 3), "wall_s_with_diagnostics": round(wall, 3),
+                     "state_ok": bool(ok)}
+    ke_err = max(abs(a[1] - b[1]) / abs(a[1]) for a, b in zip(hist["fp64"], hist["fp32_bulk"]))
+    en_err = max(abs(a[2] - b[2]) / abs(a[2]) for a, b in zip(hist["fp64"], hist["fp32_bulk"]))
+    out["fp32_bulk"]["ke_rel_err_max"] = ke_err
+    out["fp32_bulk"]["enstrophy_rel_err_max"] = en_err
+    out["fp32_bulk"]["sustained_speedup_vs_fp64"] = round(out["fp32_bulk"]["sustained_Mcell_updates_per_s_stage"] /
+                                                          out["fp64"]["sustained_Mcell_updates_per_s_stage"], 3)
+    out["history"] = {"step": [r[0] for r in hist["fp64"]],
+                      "ke_fp64": [r[1] for r in hist["fp64"]], "ke_fp32_bulk": [r[1] for r in hist["fp32_bulk"]],
+                      "enstrophy_fp64": [r[2] for r in hist["fp64"]],
+                      "enstrophy_fp32_bulk": [r[2] for r in hist["fp32_bulk"]]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-me4", dest="me4", action="store_false",
                     help="skip the ME4Base-viscous variant of the workload")
+    ap.add_argument("--history-steps", type=int, default=500,
+                    help="steps of the configuration-2 KE/enstrophy history in both forms (0: skip it)")
     ap.add_argument("--les-n", type=int, default=384,
                     help="interior size of the LES decaying-turbulence leg (0: skip it)")
-    ap.add_argument("--les-steps", type=int, default=3)
+    ap.add_argument("--les-steps", type=int, default=30)
     ap.add_argument("--os-cases", type=int, default=64,
                     help="cases of the batched 2-D oblique-shock leg (0: skip it)")
     ap.add_argument("--os-steps", type=int, default=2000)
@@ -343,27 +476,39 @@ def main():
     r64 = run_form(spec, args.steps, args.warmup, True)
     r32 = run_form(spec32, args.steps, args.warmup, True)
 
-    cell_stages = spec.interior_cells * spec.stages * args.steps
+    cells = spec.interior_cells
+    cell_stages = cells * spec.stages * args.steps
     value = cell_stages / r64["wall_s"] / 1e6
     value32 = cell_stages / r32["wall_s"] / 1e6
     ke64, ke32 = r64["diag"]["ke"], r32["diag"]["ke"]
 
-    # dominant kernel of the FP64 form, measured live with HIP events
+    # The roofline of the dominant kernel, as the algorithmic bytes of the
+    # work one launch does (106.7 B per cell-stage of RK3: the compulsory FP64
+    # state reads and writes, primitives and residual recomputed on chip, per
+    # cell of the launch) over its average launch time, measured live with HIP
+    # events on the library's stream.
+    B_alg = 320.0 / 3.0
     model = kernel_model(spec, ops)
     kt = r64["kernels"]
     dom = max(kt, key=lambda k: kt[k][0])
     avg_ms = kt[dom][0] / kt[dom][1]
-    achieved = model[dom]["bytes"] / (avg_ms * 1e-3) / 1e9
+    alg_bytes = B_alg * cells
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = measured_traffic(args.n)
     kernels = {}
     for k, (tot, n) in kt.items():
         a = tot / n
         kernels[k] = {"avg_ms": round(a, 4), "share": round(tot / sum(v[0] for v in kt.values()), 4),
-                      "alg_GBs": round(model[k]["bytes"] / (a * 1e-3) / 1e9, 1),
+                      "compulsory_GBs": round(model[k]["bytes"] / (a * 1e-3) / 1e9, 1),
+                      "measured_traffic_GBs": round(traffic[k] / (a * 1e-3) / 1e9, 1) if traffic and k in traffic
+                      else None,
                       "TFLOPs": round(model[k]["ops"] / (a * 1e-3) / 1e12, 2)}
     F_alg = ops["per_cell_stage_smooth"]
-    B_alg = 320.0 / 3.0
     stage_rate = value * 1e6
-    bound_rate = min(HBM_PEAK_GBS * 1e9 / B_alg, FP64_PEAK_TFLOPS * 1e12 / F_alg)
+    hbm_rate = HBM_PEAK_GBS * 1e9 / B_alg
+    fp64_rate = FP64_PEAK_TFLOPS * 1e12 / F_alg
+    bound_rate = min(hbm_rate, fp64_rate)
+    stage_ms = r64["wall_s"] * 1e3 / (args.steps * spec.stages)
 
     line = {
         "metric": "Mcell-updates/s per RK stage, FP32-bulk vs FP64 plain; p50/p99 solve latency",
@@ -387,10 +532,19 @@ def main():
                       "ke_rel_err_vs_fp64": abs(ke32 - ke64) / abs(ke64)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC_256.get(dom) if args.n == 256 else None,
-                     "alg_bytes_per_launch": model[dom]["bytes"], "avg_launch_ms": round(avg_ms, 4)},
-        "stage_roofline": {"F_alg_per_cell_stage": F_alg, "B_alg_per_cell_stage": round(B_alg, 1),
-                           "binding": "fp64" if FP64_PEAK_TFLOPS * 1e12 / F_alg < HBM_PEAK_GBS * 1e9 / B_alg else "hbm",
+                     "traffic": traffic.get(dom) if traffic else None,
+                     "alg_bytes_per_cell_stage": round(B_alg, 2), "cells_per_launch": cells,
+                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_ms, 4),
+                     "kernel_compulsory_bytes_per_launch": model[dom]["bytes"],
+                     "kernel_compulsory_GBs": round(model[dom]["bytes"] / (avg_ms * 1e-3) / 1e9, 1),
+                     "traffic_source": "ouxsbli_amd/flops/traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
+                                       "tagged with the kernel sources' SHA-256)" if traffic else
+                                       "no measurement on the current kernel sources"},
+        "stage_roofline": {"stage_ms": round(stage_ms, 4),
+                           "F_alg_per_cell_stage": F_alg, "B_alg_per_cell_stage": round(B_alg, 1),
+                           "hbm_frac": round(stage_rate / hbm_rate, 4),
+                           "fp64_frac": round(stage_rate / fp64_rate, 4),
+                           "binding": "fp64" if fp64_rate < hbm_rate else "hbm",
                            "bound_Mcell_stage_s": round(bound_rate / 1e6, 1),
                            "frac": round(stage_rate / bound_rate, 4)},
         "kernels": kernels,
@@ -403,6 +557,8 @@ def main():
         v_me4 = spec_me4.interior_cells * spec_me4.stages * args.steps / r_me4["wall_s"] / 1e6
         line["viscous_me4base"] = {"value": round(v_me4, 1), "ms_per_step": round(r_me4["wall_s"] * 1e3 / args.steps, 3),
                                    "form": "FP64 plain, the same workload with the ME4Base viscous stencil"}
+    if args.history_steps > 0:
+        line["accuracy_history"] = history_leg(args.n, args.history_steps, 10)
     if args.les_n > 0:
         line["les_dhit"] = les_leg(args.les_n, args.les_steps)
     if args.os_cases > 0:
@@ -411,8 +567,7 @@ def main():
     if args.serve_requests > 0:
         line["serving"] = serving_leg(args.serve_requests, value)
     if not args.no_cpu:
-        small = cases.config2_spec(64, "fp64")
-        line["cpu_baseline"] = cpu_baseline(small, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
 
 

@@ -148,6 +148,12 @@ int ox_step(ox_solver* s, int32_t nsteps);
  * Parity hook; not used by ox_step. */
 int ox_rhs(ox_solver* s, double* R);
 
+/* Host helper of the reference's .d output: the sum, in order, of n REAL(8)
+ * terms into a REAL(4) accumulator (each addition in double precision, the
+ * running sum rounded to single), as print_KE / print_entropy accumulate
+ * (src/print.f90:60-67, :101-109, with io = 4).  No device work. */
+int ox_real4_sum(const double* terms, int64_t n, float* out);
+
 /* Interior diagnostics of the current state on the device.  Replaces the
  * host-side FP32 print_KE / print_entropy (src/print.f90:42-122). */
 int ox_diagnostics(ox_solver* s, ox_diag* out);
@@ -357,10 +363,16 @@ int ox_batch_admit2d(ox_batch* b, const ox2d_config* cfg, const double* Q, int32
 #define OX_IC_TAYLOR_GREEN 1 /* NSTGV set_init: p, rho, u, v (w = 0) at the cell centres            */
 #define OX_IC_SHEAR_LAYER 2  /* double tanh shear layer in y with seeded velocity noise (no reference
                                 counterpart: the Kelvin-Helmholtz request stream's field)           */
+#define OX_IC_SPECTRAL 3     /* velocity synthesised from spectral modes (replaces the inverse FFT of
+                                3D_solver/DHIT/set_init_dhit.f90:170-179): on a cubic n^3 interior,
+                                u(x) = sum over kx = 0..K, ky, kz = -K..K of w_kx Re(H(k) e^{i k.x}) / n^3
+                                at x = 2 pi (interior index) / n, w_0 = 1, w_kx = 2 otherwise (the
+                                inverse real transform of the half spectrum), then scaled so that
+                                sqrt(<|u|^2>) = V0; uniform density rho0 and pressure p0            */
 
 typedef struct ox_ic {
   int32_t kind;     /* OX_IC_*                                                                  */
-  int32_t reserved; /* 0                                                                        */
+  int32_t kmax;     /* spectral: K of the mode box (K < n/2); 0 for the other kinds             */
   uint64_t seed;    /* shear layer: noise seed                                                  */
   double L0;        /* Taylor-Green: length scale of sin(x/L0), cos(2x/L0), ...                 */
   double V0;        /* Taylor-Green: V0; shear layer: U, the velocity difference across a layer */
@@ -369,6 +381,9 @@ typedef struct ox_ic {
   double T0;        /* Taylor-Green: temperature, rho = p / (R T0)                              */
   double delta;     /* shear layer: thickness d of tanh((y - Ly/4) / d)                         */
   double noise;     /* shear layer: amplitude of the uniform [-1, 1) velocity noise             */
+  const double* modes; /* spectral: host array H[3][K+1][2K+1][2K+1] of complex coefficients
+                          (re, im interleaved; component, kx, ky + K, kz + K); copied during the
+                          call                                                                     */
 } ox_ic;
 
 /* Builds the initial state of `ic` in the handle (interior at the cell

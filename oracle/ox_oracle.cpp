@@ -898,6 +898,15 @@ int oxo_get_primitives(void* hv, double* Qp, double* T, double* mu, double* sens
   return 0;
 }
 
+// SGS fields of the last RHS evaluation of an LES case: mut = rho * nu_t and
+// qc2 (calc_les.f90:96-125, after set_bc_mut_common), (nx,ny,nz) each.
+int oxo_get_les(void* hv, double* mut, double* qc2) {
+  auto& o = static_cast<oxo_handle*>(hv)->o;
+  if (mut) std::memcpy(mut, o.mut.data(), o.ncell() * sizeof(double));
+  if (qc2) std::memcpy(qc2, o.qc2.data(), o.ncell() * sizeof(double));
+  return 0;
+}
+
 int oxo_diagnostics(void* hv, ox_diag* d) {
   static_cast<oxo_handle*>(hv)->o.diagnostics(*d);
   return 0;

@@ -88,6 +88,7 @@ class Ox2dConfig(ctypes.Structure):
 
 OX_IC_TAYLOR_GREEN = 1
 OX_IC_SHEAR_LAYER = 2
+OX_IC_SPECTRAL = 3
 
 
 class OxIc(ctypes.Structure):
@@ -95,7 +96,7 @@ class OxIc(ctypes.Structure):
 
     _fields_ = [
         ("kind", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("kmax", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
         ("L0", ctypes.c_double),
         ("V0", ctypes.c_double),
@@ -104,6 +105,7 @@ class OxIc(ctypes.Structure):
         ("T0", ctypes.c_double),
         ("delta", ctypes.c_double),
         ("noise", ctypes.c_double),
+        ("modes", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -177,6 +179,7 @@ EXPORTS = (
     "ox_step",
     "ox_rhs",
     "ox_diagnostics",
+    "ox_real4_sum",
     "ox_check_state",
     "ox_synchronize",
     "ox_last_step_ms",

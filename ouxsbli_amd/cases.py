@@ -9,6 +9,7 @@ filled periodically) ready for ``ox_set_state``.
 """
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import math
 from typing import Optional
@@ -114,6 +115,14 @@ class CaseSpec:
         if self.ic == "khi_tanh":
             return _abi.OxIc(kind=_abi.OX_IC_SHEAR_LAYER, seed=int(self.seed), V0=self.V0, rho0=self.RHO0,
                              p0=self.p0, delta=self.Ly / 28.0, noise=1e-3 * self.V0)
+        if self.ic == "dhit":
+            n = self.nx - self.order
+            H = dhit_modes(n, int(self.seed))
+            ic = _abi.OxIc(kind=_abi.OX_IC_SPECTRAL, kmax=H.shape[1] - 1, seed=int(self.seed), V0=self.V0,
+                           rho0=self.RHO0, p0=self.p0,
+                           modes=H.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+            ic._modes = H  # the host array lives as long as the descriptor
+            return ic
         return None
 
     def state_shape(self):
@@ -308,10 +317,19 @@ def khi_state(spec: CaseSpec) -> np.ndarray:
 # built mode by mode in the two-vector frame normal to k (the construction of
 # 3D_solver/DHIT/set_init_dhit.f90:96-169, Rogallo 1981), here with the
 # spectrum E(k) ~ k^4 exp(-2 (k/k0)^2), k0 = 4, on the half spectrum of a real
-# FFT, inverted with numpy's irfftn and scaled to the turbulent Mach number
-# Mt = sqrt(<|u|^2>)/c0 = u' sqrt(3)/c0.  Uniform density and temperature in
-# physical units (the viscosity follows Sutherland's law).
+# transform, and scaled to the turbulent Mach number Mt = sqrt(<|u|^2>)/c0 =
+# u' sqrt(3)/c0.  Uniform density and temperature in physical units (the
+# viscosity follows Sutherland's law).  The three random phases of a mode are
+# a counter-based draw keyed by (seed, kx, ky, kz), so a mode's coefficients
+# do not depend on the grid: the host form inverts the whole half spectrum
+# with an inverse real FFT, the library (ox_init_state, OX_IC_SPECTRAL)
+# synthesises the field on the device from the modes with |k| <= DHIT_KMAX,
+# beyond which every amplitude is below 1e-20 of the largest.
 # ---------------------------------------------------------------------------
+DHIT_K0 = 4.0
+DHIT_KMAX = 28
+
+
 def dhit_spec(n: int, precision: str = "fp32_bulk", seed: int = 20261203, Mt: float = 0.3,
               cfl: float = 0.3, order: int = 6) -> CaseSpec:
     N = n + order
@@ -325,21 +343,34 @@ def dhit_spec(n: int, precision: str = "fp32_bulk", seed: int = 20261203, Mt: fl
                     Lx=L, Ly=L, Lz=L, ic="dhit", V0=urms, RHO0=RHO0, T0=T0, p0=RHO0 * R_GAS * T0, seed=seed)
 
 
-def dhit_velocity(n: int, seed: int, k0: float = 4.0):
-    """Solenoidal random-phase velocity (3, n, n, n) in (z, y, x) order with
-    E(k) ~ k^4 exp(-2 (k/k0)^2) (integer wavenumbers of the 2*pi box), not
-    yet scaled."""
-    rng = np.random.default_rng(seed)
-    k1 = np.fft.fftfreq(n, d=1.0 / n)
-    kr = np.fft.rfftfreq(n, d=1.0 / n)
-    KZ, KY, KX = np.meshgrid(k1, k1, kr, indexing="ij")
+def _mode_phases(seed: int, KX, KY, KZ):
+    """The three phases in [0, 2 pi) of each mode (integer wavenumbers, kx >= 0):
+    SplitMix64 of (seed + 1) * 0x9E3779B97F4A7C15 + 3 id + c with the mode id
+    ((kz + 2^20) 2^21 + (ky + 2^20)) 2^21 + kx, top 53 bits scaled to [0, 1)."""
+    base = ((int(seed) + 1) * _SM_GAMMA) & _U64
+    off = np.int64(1 << 20)
+    mid = ((KZ.astype(np.int64) + off) * np.int64(1 << 21) + (KY.astype(np.int64) + off)) * np.int64(1 << 21) + \
+        KX.astype(np.int64)
+    mid = mid.astype(np.uint64)
+    out = []
+    with np.errstate(over="ignore"):
+        for c in range(3):
+            key = np.uint64(base) + (np.uint64(3) * mid + np.uint64(c))
+            out.append((splitmix64(key) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53 * (2.0 * np.pi))
+    return out
+
+
+def _dhit_coeffs(KX, KY, KZ, n: int, seed: int, k0: float = DHIT_K0):
+    """Spectral coefficients (u_hat, v_hat, w_hat) of the modes on the given
+    integer wavenumber arrays (kx >= 0); zero at k = 0 and at the Nyquist
+    wavenumbers of an n^3 grid."""
     kmag = np.sqrt(KX * KX + KY * KY + KZ * KZ)
     k12 = np.sqrt(KX * KX + KY * KY)
-    nyq = (np.abs(KX) == n // 2) | (np.abs(KY) == n // 2) | (np.abs(KZ) == n // 2)  # no Nyquist modes
+    nyq = (np.abs(KX) == n // 2) | (np.abs(KY) == n // 2) | (np.abs(KZ) == n // 2)
+    ph = _mode_phases(seed, KX, KY, KZ)
     with np.errstate(divide="ignore", invalid="ignore"):
         E = kmag ** 4 * np.exp(-2.0 * (kmag / k0) ** 2)
         amp = np.where((kmag > 0) & ~nyq, np.sqrt(E / (2.0 * np.pi * kmag * kmag)), 0.0)
-        ph = rng.uniform(0.0, 2.0 * np.pi, (3,) + kmag.shape)
         a = amp * np.exp(1j * ph[0]) * np.cos(ph[2])
         b = amp * np.exp(1j * ph[1]) * np.sin(ph[2])
         inplane = k12 > 0
@@ -348,12 +379,37 @@ def dhit_velocity(n: int, seed: int, k0: float = 4.0):
         ux = np.where(inplane, (KY / s12) * a + (KX / s12) * (KZ / sk) * b, a)
         uy = np.where(inplane, -(KX / s12) * a + (KY / s12) * (KZ / sk) * b, 0.0)
         uz = np.where(inplane, -(k12 / sk) * b, 0.0)
+    return ux, uy, uz
+
+
+def dhit_velocity(n: int, seed: int, k0: float = DHIT_K0):
+    """Solenoidal random-phase velocity (3, n, n, n) in (z, y, x) order with
+    E(k) ~ k^4 exp(-2 (k/k0)^2) (integer wavenumbers of the 2*pi box), not
+    yet scaled: the inverse real FFT of the whole half spectrum."""
+    k1 = np.fft.fftfreq(n, d=1.0 / n)
+    kr = np.fft.rfftfreq(n, d=1.0 / n)
+    KZ, KY, KX = np.meshgrid(k1, k1, kr, indexing="ij")
+    ux, uy, uz = _dhit_coeffs(KX, KY, KZ, n, seed, k0)
     try:
         import scipy.fft as sfft
         inv = lambda c: sfft.irfftn(c, s=(n, n, n), workers=-1)
     except ImportError:
         inv = lambda c: np.fft.irfftn(c, s=(n, n, n))
     return np.stack([inv(c) for c in (ux, uy, uz)])
+
+
+def dhit_modes(n: int, seed: int, kmax: int = DHIT_KMAX, k0: float = DHIT_K0) -> np.ndarray:
+    """The modes the device synthesis sums: complex array (3, kx, ky, kz)
+    over kx = 0..K, ky, kz = -K..K with K = min(kmax, n/2 - 1), zero outside
+    the ball |k| <= kmax; the same coefficients dhit_velocity inverts."""
+    K = min(kmax, n // 2 - 1)
+    kx = np.arange(0, K + 1, dtype=np.float64)
+    ks = np.arange(-K, K + 1, dtype=np.float64)
+    KX, KY, KZ = np.meshgrid(kx, ks, ks, indexing="ij")
+    ux, uy, uz = _dhit_coeffs(KX, KY, KZ, n, seed, k0)
+    H = np.stack([ux, uy, uz]).astype(np.complex128)
+    H[:, (KX * KX + KY * KY + KZ * KZ) > kmax * kmax] = 0.0
+    return np.ascontiguousarray(H)
 
 
 def dhit_state(spec: CaseSpec) -> np.ndarray:

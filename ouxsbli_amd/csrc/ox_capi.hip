@@ -240,6 +240,16 @@ int ox_host_free(void* ptr) {
 
 const char* ox_version(void) { return "ouxsbli-mi355x 1 gfx950"; }
 
+int ox_real4_sum(const double* terms, int64_t n, float* out) {
+  if (!out || (n > 0 && !terms) || n < 0) return ox_fail(OX_EINVAL, "null argument or negative count");
+  // a REAL(4) accumulator: each REAL(8) term is added in double precision and
+  // the sum rounded back to single, in order
+  float acc = 0.0f;
+  for (int64_t m = 0; m < n; ++m) acc = (float)((double)acc + terms[m]);
+  *out = acc;
+  return OX_OK;
+}
+
 int ox_create(const ox_config* cfg, ox_solver** out) {
   if (!cfg || !out) return ox_fail(OX_EINVAL, "null argument");
   *out = nullptr;

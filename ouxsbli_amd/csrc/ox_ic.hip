@@ -16,7 +16,9 @@
 // tanh round differently from the host's (a few ulp).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
+#include <vector>
 
 #include "ox2d_core.hpp"
 #include "ox_solver.hpp"
@@ -96,6 +98,153 @@ __global__ __launch_bounds__(256) void k_ic(double* __restrict__ QJ, Geom g, IcA
   for (int l = 0; l < 5; ++l) QJ[i + (size_t)g.nx * (l + 5 * ((size_t)j + (size_t)g.ny * k))] = q[l] / a.J;
 }
 
+// ---------------------------------------------------------------------------
+// OX_IC_SPECTRAL: the inverse real transform of a band-limited half spectrum,
+// done as three pruned partial sums over the mode box (|k| <= K in every
+// direction) instead of full-size FFTs:
+//   G1[c][kx][ky][z] = sum_kz H[c][kx][ky][kz] e^{2 pi i kz z / n}
+//   G2[c][kx][y][z]  = sum_ky G1[c][kx][ky][z] e^{2 pi i ky y / n}
+//   u_c(x, y, z)     = (Re G2[c][0][y][z] + 2 sum_{kx>=1} Re(G2[c][kx][y][z] e^{2 pi i kx x / n})) / n^3
+// Twiddles come from a table of e^{2 pi i r / n}, r = k m mod n, so every
+// angle is reduced exactly.  Two passes over the cells: the first reduces
+// sum |u|^2 (per-block Neumaier partials, merged on the host in block order),
+// the second recomputes u, scales it to the requested rms and writes QJ.
+// ---------------------------------------------------------------------------
+struct Cplx {
+  double re, im;
+};
+
+__global__ __launch_bounds__(256) void k_spec_twiddle(Cplx* __restrict__ tw, int n) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  double sn, cs;
+  sincospi(2.0 * (double)r / (double)n, &sn, &cs);
+  tw[r] = Cplx{cs, sn};
+}
+
+__device__ __forceinline__ int tw_index(int k, int m, int n) {
+  const long long r = ((long long)k * m) % n;
+  return (int)(r < 0 ? r + n : r);
+}
+
+// G1: one thread per (z, ky, kx, c); H is [3][K+1][2K+1][2K+1] (c, kx, ky, kz)
+__global__ __launch_bounds__(256) void k_spec_z(const Cplx* __restrict__ H, const Cplx* __restrict__ tw,
+                                                Cplx* __restrict__ G1, int K, int n) {
+  const int nk = 2 * K + 1;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long total = 3LL * (K + 1) * nk * n;
+  if (t >= total) return;
+  const int z = (int)(t % n);
+  const long long row = t / n;  // (c, kx, ky)
+  const Cplx* h = H + row * nk;
+  double re = 0.0, im = 0.0;
+  for (int a = 0; a < nk; ++a) {
+    const Cplx e = tw[tw_index(a - K, z, n)];
+    re += h[a].re * e.re - h[a].im * e.im;
+    im += h[a].re * e.im + h[a].im * e.re;
+  }
+  G1[t] = Cplx{re, im};  // [c][kx][ky][z]
+}
+
+// G2: one thread per (z, y, kx, c)
+__global__ __launch_bounds__(256) void k_spec_y(const Cplx* __restrict__ G1, const Cplx* __restrict__ tw,
+                                                Cplx* __restrict__ G2, int K, int n) {
+  const int nk = 2 * K + 1;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long total = 3LL * (K + 1) * n * n;
+  if (t >= total) return;
+  const int z = (int)(t % n);
+  const int y = (int)((t / n) % n);
+  const long long ck = t / ((long long)n * n);  // (c, kx)
+  const Cplx* g = G1 + ck * nk * n;
+  double re = 0.0, im = 0.0;
+  for (int b = 0; b < nk; ++b) {
+    const Cplx v = g[(long long)b * n + z];
+    const Cplx e = tw[tw_index(b - K, y, n)];
+    re += v.re * e.re - v.im * e.im;
+    im += v.re * e.im + v.im * e.re;
+  }
+  G2[t] = Cplx{re, im};  // [c][kx][y][z]
+}
+
+// u, v, w at grid point (x, y, z) from G2 (layout [c][kx][y][z])
+__device__ __forceinline__ void spec_velocity(const Cplx* __restrict__ G2, const Cplx* __restrict__ tw, int K, int n,
+                                              int x, int y, int z, double inv_n3, double* u) {
+  const long long plane = (long long)n * n;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const Cplx* g = G2 + (long long)c * (K + 1) * plane + (long long)y * n + z;
+    double acc = g[0].re;
+    double s = 0.0;
+    for (int kx = 1; kx <= K; ++kx) {
+      const Cplx v = g[(long long)kx * plane];
+      const Cplx e = tw[tw_index(kx, x, n)];
+      s += v.re * e.re - v.im * e.im;
+    }
+    u[c] = (acc + 2.0 * s) * inv_n3;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_spec_rms(const Cplx* __restrict__ G2, const Cplx* __restrict__ tw, int K,
+                                                  int n, double inv_n3, double* __restrict__ partial) {
+  __shared__ double ss[256], sc[256];
+  const long long N = (long long)n * n * n;
+  double sum = 0.0, comp = 0.0;
+  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < N; t += (long long)gridDim.x * 256) {
+    const int x = (int)(t % n), y = (int)((t / n) % n), z = (int)(t / ((long long)n * n));
+    double u[3];
+    spec_velocity(G2, tw, K, n, x, y, z, inv_n3, u);
+    const double q = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double tt = sum + q;
+    if (__builtin_fabs(sum) >= __builtin_fabs(q)) comp += (sum - tt) + q;
+    else comp += (q - tt) + sum;
+    sum = tt;
+  }
+  const int tid = threadIdx.x;
+  ss[tid] = sum;
+  sc[tid] = comp;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (tid < off) {
+      const double a = ss[tid], b = ss[tid + off];
+      const double tt = a + b;
+      double c = sc[tid] + sc[tid + off];
+      if (__builtin_fabs(a) >= __builtin_fabs(b)) c += (a - tt) + b;
+      else c += (b - tt) + a;
+      ss[tid] = tt;
+      sc[tid] = c;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    partial[2 * blockIdx.x] = ss[0];
+    partial[2 * blockIdx.x + 1] = sc[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_spec_state(double* __restrict__ QJ, Geom g, const Cplx* __restrict__ G2,
+                                                    const Cplx* __restrict__ tw, int K, int n, double inv_n3,
+                                                    double scale, double rho, double pe, double J) {
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * 4 + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= n || y >= n) return;
+  double u[3];
+  spec_velocity(G2, tw, K, n, x, y, z, inv_n3, u);
+  u[0] *= scale;
+  u[1] *= scale;
+  u[2] *= scale;
+  double q[5];
+  q[0] = rho;
+  q[1] = rho * u[0];
+  q[2] = rho * u[1];
+  q[3] = rho * u[2];
+  q[4] = pe + 0.5 * rho * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  const int i = g.h + x, j = g.h + y, k = g.h + z;
+#pragma unroll
+  for (int l = 0; l < 5; ++l) QJ[i + (size_t)g.nx * (l + 5 * ((size_t)j + (size_t)g.ny * k))] = q[l] / J;
+}
+
 }  // namespace ox
 
 namespace ox2 {
@@ -114,10 +263,102 @@ __global__ __launch_bounds__(256) void k_ic_os(double* __restrict__ QJ, int nx, 
 }
 }  // namespace ox2
 
+namespace {
+// The fresh state's bookkeeping, as after ox_set_qj: RK4 accumulator zero,
+// primitives stale (a fill kernel, not a memset, so the call may be captured).
+int finish_init(ox_solver* s) {
+  ox::Core& C = s->C;
+  const size_t n = state_elems(C.g);
+  if (s->Rs) {
+    ox::k_zero<<<(unsigned)((n + 255) / 256), 256, 0, C.stream>>>(s->Rs, (long long)n);
+    OX_HIP(hipGetLastError());
+  }
+  C.prim_for = nullptr;
+  OX_HIP(hipStreamSynchronize(C.stream));
+  return OX_OK;
+}
+
+int init_spectral(ox_solver* s, const ox_ic* ic) {
+  ox::Core& C = s->C;
+  const ox::Geom& g = C.g;
+  const ox_config& c = C.cfg;
+  const int n = g.ix, K = ic->kmax;
+  if (g.iy != n || g.iz != n) return ox_fail(OX_EINVAL, "spectral initial state needs a cubic interior");
+  if (!ic->modes || K < 1 || 2 * K + 1 > n) return ox_fail(OX_EINVAL, "spectral modes missing or kmax not < n/2");
+  if (!(ic->V0 > 0.0) || !(ic->rho0 > 0.0) || !(ic->p0 > 0.0))
+    return ox_fail(OX_EINVAL, "spectral initial state needs V0, rho0, p0 > 0");
+  OX_HIP(hipSetDevice(c.device));
+  const int nk = 2 * K + 1;
+  const size_t nH = 3ull * (K + 1) * nk * nk, nG1 = 3ull * (K + 1) * nk * n, nG2 = 3ull * (K + 1) * n * n;
+  const int nb = 1024;
+  ox::Cplx *H = nullptr, *tw = nullptr, *G1 = nullptr, *G2 = nullptr;
+  double* part = nullptr;
+  auto release = [&]() {
+    (void)hipFree(H);
+    (void)hipFree(tw);
+    (void)hipFree(G1);
+    (void)hipFree(G2);
+    (void)hipFree(part);
+  };
+  if (hipMalloc(&H, nH * sizeof(ox::Cplx)) != hipSuccess || hipMalloc(&tw, n * sizeof(ox::Cplx)) != hipSuccess ||
+      hipMalloc(&G1, nG1 * sizeof(ox::Cplx)) != hipSuccess || hipMalloc(&G2, nG2 * sizeof(ox::Cplx)) != hipSuccess ||
+      hipMalloc(&part, 2 * nb * sizeof(double)) != hipSuccess) {
+    release();
+    return ox_fail(OX_ENOMEM, "spectral initial state: device allocation failed");
+  }
+  int rc = OX_OK;
+  std::vector<double> hp(2 * nb);
+  do {
+    if (hipMemcpyAsync(H, ic->modes, nH * sizeof(ox::Cplx), hipMemcpyHostToDevice, C.stream) != hipSuccess) {
+      rc = ox_fail(OX_EDEVICE, "spectral modes upload failed");
+      break;
+    }
+    ox::k_spec_twiddle<<<(n + 255) / 256, 256, 0, C.stream>>>(tw, n);
+    ox::k_spec_z<<<(unsigned)((nG1 + 255) / 256), 256, 0, C.stream>>>(H, tw, G1, K, n);
+    ox::k_spec_y<<<(unsigned)((nG2 + 255) / 256), 256, 0, C.stream>>>(G1, tw, G2, K, n);
+    const double inv_n3 = 1.0 / ((double)n * (double)n * (double)n);
+    ox::k_spec_rms<<<nb, 256, 0, C.stream>>>(G2, tw, K, n, inv_n3, part);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, C.stream) != hipSuccess ||
+        hipStreamSynchronize(C.stream) != hipSuccess) {
+      rc = ox_fail(OX_EDEVICE, "spectral synthesis failed");
+      break;
+    }
+    double sum = 0.0, comp = 0.0;
+    for (int b = 0; b < nb; ++b) {
+      const double x = hp[2 * b], t = sum + x;
+      if (std::fabs(sum) >= std::fabs(x)) comp += (sum - t) + x;
+      else comp += (x - t) + sum;
+      sum = t;
+      comp += hp[2 * b + 1];
+    }
+    const double mean = (sum + comp) / ((double)n * (double)n * (double)n);
+    if (!(mean > 0.0)) {
+      rc = ox_fail(OX_EINVAL, "spectral modes give a zero velocity field");
+      break;
+    }
+    const double scale = ic->V0 / std::sqrt(mean);
+    const dim3 grid((n + 63) / 64, (n + 3) / 4, n);
+    ox::k_spec_state<<<grid, dim3(64, 4, 1), 0, C.stream>>>(s->QJ, g, G2, tw, K, n, inv_n3, scale, ic->rho0,
+                                                             ic->p0 / (c.gamma - 1.0), g.jac);
+    if (hipGetLastError() != hipSuccess) {
+      rc = ox_fail(OX_EDEVICE, "spectral state kernel failed");
+      break;
+    }
+    ox::launch_ghost_fill(C, s->QJ);
+    if (hipStreamSynchronize(C.stream) != hipSuccess) rc = ox_fail(OX_EDEVICE, "spectral initial state failed");
+  } while (false);
+  release();
+  if (rc != OX_OK) return rc;
+  return finish_init(s);
+}
+}  // namespace
+
 extern "C" {
 
 int ox_init_state(ox_solver* s, const ox_ic* ic) {
   if (!s || !ic) return ox_fail(OX_EINVAL, "null argument");
+  if (ic->kind == OX_IC_SPECTRAL) return init_spectral(s, ic);
   if (ic->kind != OX_IC_TAYLOR_GREEN && ic->kind != OX_IC_SHEAR_LAYER)
     return ox_fail(OX_EINVAL, "unknown initial condition kind");
   if (ic->kind == OX_IC_TAYLOR_GREEN && (ic->L0 <= 0.0 || ic->T0 <= 0.0))
@@ -141,16 +382,7 @@ int ox_init_state(ox_solver* s, const ox_ic* ic) {
   OX_HIP(hipGetLastError());
   ox::launch_ghost_fill(C, s->QJ);
   OX_HIP(hipGetLastError());
-  const size_t n = state_elems(g);
-  // the fresh state's bookkeeping, as after ox_set_qj: RK4 accumulator zero,
-  // primitives stale (a fill kernel, not a memset, so the call may be captured)
-  if (s->Rs) {
-    ox::k_zero<<<(unsigned)((n + 255) / 256), 256, 0, C.stream>>>(s->Rs, (long long)n);
-    OX_HIP(hipGetLastError());
-  }
-  C.prim_for = nullptr;
-  OX_HIP(hipStreamSynchronize(C.stream));
-  return OX_OK;
+  return finish_init(s);
 }
 
 int ox2d_init_state(ox2d_solver* s) {

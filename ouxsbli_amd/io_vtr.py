@@ -22,9 +22,12 @@ import numpy as np
 def fortran_e(x: float, w: int = 12, d: int = 4) -> str:
     """Fortran ``Ew.d`` edit descriptor: 0.dddd mantissa, E+xx exponent."""
     x = float(x)
-    if x == 0.0 or not math.isfinite(x):
-        body = f"0.{'0' * d}E+00" if x == 0.0 else str(x)
-        return body.rjust(w)
+    if math.isnan(x):
+        return "NaN".rjust(w)
+    if math.isinf(x):
+        return ("Infinity" if x > 0 else "-Infinity").rjust(w)
+    if x == 0.0:
+        return f"0.{'0' * d}E+00".rjust(w)
     sign = "-" if x < 0 else ""
     a = abs(x)
     e = math.floor(math.log10(a)) + 1
@@ -37,7 +40,8 @@ def fortran_e(x: float, w: int = 12, d: int = 4) -> str:
         mant *= 10.0
         e -= 1
     digits = f"{mant:.{d}f}"[1:]  # ".dddd"
-    es = f"E{'+' if e >= 0 else '-'}{abs(e):02d}"
+    # a two-digit exponent takes the E; a three-digit one replaces it
+    es = f"E{'+' if e >= 0 else '-'}{abs(e):02d}" if abs(e) <= 99 else f"{'+' if e >= 0 else '-'}{abs(e):03d}"
     return (sign + "0" + digits + es).rjust(w)
 
 
@@ -126,22 +130,35 @@ def read_vtr(path):
     return x, y, z, rho.reshape(nk, nj, ni), p.reshape(nk, nj, ni), vel.reshape(nk, nj, ni, 3)
 
 
-def _fp32_mean(terms: np.ndarray, count: int) -> np.float32:
-    # sequential accumulation into a REAL(4) sum, then / dble(count)
-    acc = np.cumsum(terms.astype(np.float32).ravel(), dtype=np.float32)[-1]
-    return np.float32(np.float64(acc) / float(count))
+def real4_mean(terms: np.ndarray, count: int) -> np.float32:
+    """REAL(4) accumulation of REAL(8) terms in order, then / dble(count)
+    (src/print.f90:60-68, :101-110): each addition in double precision with
+    the running sum rounded to single, done by the library's host helper."""
+    import ctypes
+    from . import _native
+    t = np.ascontiguousarray(terms, dtype=np.float64).ravel()
+    acc = ctypes.c_float()
+    _native.check(_native.load().ox_real4_sum(t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), t.size,
+                                              ctypes.byref(acc)))
+    return np.float32(np.float64(np.float32(acc.value)) / float(count))
 
 
 def ke_entropy(rho, p, vel, order: int, gamma: float):
-    """print_KE / print_entropy sums over the interior (src/print.f90:42-122)."""
+    """print_KE / print_entropy over the interior (src/print.f90:42-122) with
+    the reference's expression types (io = 4): the fields are REAL(4);
+    vel**2 and their sum are REAL(4), the product with 0.5d0 * rho and the
+    entropy term rho * log(p * rho**(-gamma)) are REAL(8); the sums are
+    REAL(4) accumulators; loops run i fastest, then j, then k."""
     h = order // 2
     sl = (slice(h, rho.shape[0] - h), slice(h, rho.shape[1] - h), slice(h, rho.shape[2] - h))
-    r = rho[sl].astype(np.float64)
-    vv = vel[sl].astype(np.float64)
-    pp = p[sl].astype(np.float64)
-    count = r.size
-    ke = _fp32_mean(0.5 * r * (vv[..., 0] ** 2 + vv[..., 1] ** 2 + vv[..., 2] ** 2), count)
-    ent = _fp32_mean(r * np.log(pp * r ** (-gamma)), count)
+    r32 = np.ascontiguousarray(rho[sl], dtype=np.float32)
+    p32 = np.ascontiguousarray(p[sl], dtype=np.float32)
+    v32 = np.ascontiguousarray(vel[sl], dtype=np.float32)
+    count = r32.size
+    sq = (v32[..., 0] * v32[..., 0] + v32[..., 1] * v32[..., 1]) + v32[..., 2] * v32[..., 2]  # REAL(4)
+    r64 = r32.astype(np.float64)
+    ke = real4_mean((0.5 * r64) * sq.astype(np.float64), count)
+    ent = real4_mean(r64 * np.log(p32.astype(np.float64) * (r64 ** (-gamma))), count)
     return ke, ent
 
 
@@ -154,6 +171,8 @@ class Diagnostics:
         self.s0 = np.float32(1.0)
 
     def record(self, step: int, t: float, ke: np.float32, ent: np.float32) -> None:
+        """One line of each file; t is nt * step * dt in REAL(8), stored
+        REAL(4) as the reference's `real(io) t` does."""
         if step == 0:
             self.ke0, self.s0 = ke, ent
         t32 = np.float32(t)

@@ -111,14 +111,18 @@ class ServeReport:
 
 
 def serve(requests: List[Request], hbm_budget: int, max_cases: int = 64, use_graph: bool = True,
-          autotune: bool = True, pretune: bool = True, keep_results: bool = True,
+          autotune: bool = True, pretune: bool = False, keep_results: bool = True,
           clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] = time.sleep,
           batch: Optional[Batch] = None, max_run: int = 1024) -> ServeReport:
     """Runs the stream to completion and returns the per-request timings.
 
-    ``pretune`` fills the engine's segment-length cache for every distinct
-    (specialisation, grid) of the stream before the clock starts, the way a
-    server keeps tuned plans for the shapes it serves."""
+    With ``autotune`` the engine tunes a shape's segment lengths when it first
+    admits it, inside that request's latency, and keeps the result for later
+    requests of the same shape (an engine passed in as ``batch`` keeps what
+    earlier streams tuned).  ``pretune`` instead fills the cache for every
+    shape of the stream before the clock starts, which a live server cannot
+    do; it is off by default.  The report's engine statistics cover this
+    stream only."""
     own = batch is None
     if own:
         batch = Batch(hbm_budget, max_cases=max_cases, use_graph=use_graph, autotune=autotune)
@@ -143,6 +147,7 @@ def serve(requests: List[Request], hbm_budget: int, max_cases: int = 64, use_gra
         steps = 0
         step_s = None  # running estimate of the wall time of one batch step
         host = {"admit": 0.0, "run": 0.0, "retire": 0.0}
+        stats0 = batch.stats()
         t0 = clock()
         while pending or waiting or active:
             now = clock() - t0
@@ -200,7 +205,11 @@ def serve(requests: List[Request], hbm_budget: int, max_cases: int = 64, use_gra
                 if keep_results:
                     r.result = Q
         wall = clock() - t0
-        return ServeReport(requests, wall, batch.stats(), steps, host)
+        stats = dict(batch.stats())
+        for k in ("device_ms", "graph_captures", "graph_replays", "steps", "cell_stages"):
+            if k in stats and k in stats0:
+                stats[k] = stats[k] - stats0[k]
+        return ServeReport(requests, wall, stats, steps, host)
     finally:
         if own:
             batch.close()

@@ -36,6 +36,7 @@ def lib():
         L.oxo_step.restype = ctypes.c_int
         L.oxo_get_primitives.argtypes = [P, D, D, D, D]
         L.oxo_diagnostics.argtypes = [P, ctypes.POINTER(_abi.OxDiag)]
+        L.oxo_get_les.argtypes = [P, D, D]
         L.oxo_last_error.restype = ctypes.c_char_p
         L.oxo_keep_flux.argtypes = [ctypes.POINTER(_abi.OxConfig), ctypes.c_int, D, ctypes.c_int, D]
         L.oxo_slau_flux.argtypes = [ctypes.POINTER(_abi.OxConfig), ctypes.c_int, D, ctypes.c_int,
@@ -118,6 +119,14 @@ class Oracle:
         sen = np.empty_like(T)
         lib().oxo_get_primitives(self.h, _dp(Qp), _dp(T), _dp(mu), _dp(sen))
         return Qp, T, mu, sen
+
+    def les_fields(self):
+        """(mut, qc2) of the last RHS evaluation, numpy shape (nz, ny, nx)."""
+        s = self.spec
+        mut = np.empty((s.nz, s.ny, s.nx))
+        qc2 = np.empty_like(mut)
+        lib().oxo_get_les(self.h, _dp(mut), _dp(qc2))
+        return mut, qc2
 
     def diagnostics(self):
         d = _abi.OxDiag()

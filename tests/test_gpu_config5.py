@@ -108,10 +108,16 @@ def test_max_wave_speed_exact(order):
     ref = _wave_speed_numpy(spec, QJ)
     assert dg["max_wave_speed"] == ref
     assert do["max_wave_speed"] == ref
-    # the maximum is attained where it should be: a lone fast cell moves it
+    # the maximum is attained where it should be: a lone fast cell (u = 10,
+    # the same pressure) moves it
     h = order // 2
     Q2 = Q.copy()
-    Q2[h + 5, h + 3, 1, h + 7] *= 3.0
+    c = (h + 5, h + 3, slice(None), h + 7)
+    rho = Q2[c][0]
+    u, v, w = Q2[c][1] / rho, Q2[c][2] / rho, Q2[c][3] / rho
+    p = (spec.gamma - 1.0) * (Q2[c][4] - 0.5 * rho * (u * u + v * v + w * w))
+    Q2[h + 5, h + 3, 1, h + 7] = rho * 10.0
+    Q2[h + 5, h + 3, 4, h + 7] = p / (spec.gamma - 1.0) + 0.5 * rho * (100.0 + v * v + w * w)
     with Solver(spec) as s:
         s.set_state(Q2)
         assert s.diagnostics()["max_wave_speed"] == _wave_speed_numpy(spec, s.get_qj()) > ref

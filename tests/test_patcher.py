@@ -171,3 +171,14 @@ def test_parameters_follow_fortran_integer_semantics():
     assert g["trunc"] == 2
     with pytest.raises(ValueError):
         read_parameters("integer, parameter :: nx = undefined_function(3)\n")
+
+
+def test_reference_import_names():
+    """The reference's scripts import from ``ouxsbli`` (tutorials/test.py:3-5,
+    ouxsbli/tests/test_st.py:10); those names resolve to this build."""
+    import ouxsbli
+    from ouxsbli import Case as C2
+    from ouxsbli.patcher import patch as p2
+    from ouxsbli.ic import TaylorGreenIC as T2
+    assert C2 is Case and p2 is patch and T2 is TaylorGreenIC
+    assert set(ouxsbli.__all__) == {"Case", "patch", "UniformIC", "TaylorGreenIC", "RiemannIC", "PeriodicBC"}

@@ -158,6 +158,18 @@ int ox_real4_sum(const double* terms, int64_t n, float* out);
  * host-side FP32 print_KE / print_entropy (src/print.f90:42-122). */
 int ox_diagnostics(ox_solver* s, ox_diag* out);
 
+/* The same diagnostics without any allocation: no reference counterpart.  For
+ * callers that cannot allocate (serving loops with a no-allocation rule, code
+ * that must not call hipMalloc/hipFree or allocate host memory per call).
+ * The reduction partials and the merged result live in a file-scope device
+ * array of the library (352 KiB per GPU, held until the process ends; a grid
+ * of any size fits, since the reduction never uses more than 4096 blocks), and
+ * the final merge runs on the device in the order ox_diagnostics merges on the
+ * host, so both return identical values.  Calls that overlap on one GPU (from
+ * several threads, or handles on different streams) share that array and race;
+ * serialise them. */
+int ox_diagnostics_noalloc(ox_solver* s, ox_diag* out);
+
 /* Scans the current state for non-physical cells (rho<=0, p<=0, non-finite).
  * Returns OX_OK, or OX_ESTATE with *bad_index set to the first offending
  * linear cell index (x fastest over nx,ny,nz) and *count to how many.

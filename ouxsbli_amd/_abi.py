@@ -179,6 +179,7 @@ EXPORTS = (
     "ox_step",
     "ox_rhs",
     "ox_diagnostics",
+    "ox_diagnostics_noalloc",
     "ox_real4_sum",
     "ox_check_state",
     "ox_synchronize",

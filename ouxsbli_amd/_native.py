@@ -52,6 +52,8 @@ def load() -> ctypes.CDLL:
     L.ox_diagnostics.restype = I
     L.ox_real4_sum.argtypes = [D, ctypes.c_int64, ctypes.POINTER(ctypes.c_float)]
     L.ox_real4_sum.restype = I
+    L.ox_diagnostics_noalloc.argtypes = [P, ctypes.POINTER(_abi.OxDiag)]
+    L.ox_diagnostics_noalloc.restype = I
     L.ox_check_state.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.ox_check_state.restype = I
     L.ox_synchronize.argtypes = [P]

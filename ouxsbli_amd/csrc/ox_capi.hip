@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -83,6 +84,67 @@ __global__ __launch_bounds__(256) void k_scale_to(const double* __restrict__ src
 namespace {
 thread_local std::string g_err;
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Allocation-free diagnostics (ox_diagnostics_noalloc).  The per-block
+// partials of k_diag and the merged result live in one file-scope device
+// array instead of the handle's scratch buffer, and the final merge over the
+// partials runs on the device, so a call makes no device or host allocation.
+// Each GPU holds its own copy of the array (`noalloc_scratch`, 352 KiB) until
+// the process ends; its device address is looked up once per device.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kNoAllocMaxBlocks = 4096;  // k_diag never launches more (the same cap as ox_diagnostics)
+constexpr int kDiagStride = 2 * ox::DIAG_NQ + 1;
+constexpr size_t kNoAllocDoubles = (size_t)kNoAllocMaxBlocks * kDiagStride + 8;
+static __device__ __align__(256) double noalloc_scratch[kNoAllocDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[kMaxDevices] = {};
+
+// Device address of this device's noalloc_scratch (the device must be
+// current).  Host code uses only this address, never the array's name.
+int noalloc_scratch_ptr(int dev, double** out) {
+  if (dev < 0 || dev >= kMaxDevices) return ox_fail(OX_EINVAL, "device ordinal out of range for the static scratch");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+    if (e != hipSuccess || !p)
+      return ox_fail(OX_EDEVICE, std::string("hipGetSymbolAddress(noalloc_scratch): ") + hipGetErrorString(e));
+    g_noalloc_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_noalloc_ptr[dev];
+  return OX_OK;
+}
+}  // namespace
+
+namespace ox {
+// Merges the k_diag partials of nb blocks in block order, one thread per
+// quantity (and one for the wave speed), with exactly the operations of the
+// host merge in ox_diagnostics, into out[0..DIAG_NQ] (sums) and
+// out[DIAG_NQ] (max wave speed).
+__global__ void k_diag_merge(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  const int q = threadIdx.x;
+  if (q < DIAG_NQ) {
+    double sum = 0.0, comp = 0.0;
+    for (int b = 0; b < nb; ++b) {
+      const double* p = part + (size_t)b * (2 * DIAG_NQ + 1);
+      const double x = p[2 * q];
+      const double t = sum + x;
+      if (__builtin_fabs(sum) >= __builtin_fabs(x)) comp += (sum - t) + x;
+      else comp += (x - t) + sum;
+      sum = t;
+      comp += p[2 * q + 1];
+    }
+    out[q] = sum + comp;
+  } else if (q == DIAG_NQ) {
+    double wmax = 0.0;
+    for (int b = 0; b < nb; ++b) wmax = nan_max(wmax, part[(size_t)b * (2 * DIAG_NQ + 1) + 2 * DIAG_NQ]);
+    out[DIAG_NQ] = wmax;
+  }
+}
+}  // namespace ox
 
 int ox_fail(int code, const std::string& msg) {
   g_err = msg;
@@ -479,6 +541,37 @@ int ox_diagnostics(ox_solver* s, ox_diag* out) {
   out->max_wave_speed = wmax;
   out->mass = sum[3] + comp[3];
   out->energy = sum[4] + comp[4];
+  return OX_OK;
+}
+
+int ox_diagnostics_noalloc(ox_solver* s, ox_diag* out) {
+  if (!s || !out) return ox_fail(OX_EINVAL, "null argument");
+  OX_HIP(hipSetDevice(s->C.cfg.device));
+  double* scratch = nullptr;
+  int rc = noalloc_scratch_ptr(s->C.cfg.device, &scratch);
+  if (rc != OX_OK) return rc;
+  const ox::Geom& g = s->C.g;
+  const long long n = (long long)g.ix * g.iy * g.iz;
+  int nb = (int)std::min<long long>(kNoAllocMaxBlocks, (n + ox::DIAG_BLOCK - 1) / ox::DIAG_BLOCK);
+  if (nb < 1) nb = 1;
+  if ((size_t)nb * kDiagStride + ox::DIAG_NQ + 1 > kNoAllocDoubles)
+    return ox_fail(OX_EINVAL, "diagnostics need " + std::to_string(nb) + " partials, beyond the static scratch of " +
+                                  std::to_string(kNoAllocMaxBlocks));
+  double* result = scratch + (size_t)kNoAllocMaxBlocks * kDiagStride;
+  ox::k_diag<<<nb, ox::DIAG_BLOCK, 0, s->C.stream>>>(s->QJ, g, s->C.cfg.gamma, scratch);
+  OX_HIP(hipGetLastError());
+  ox::k_diag_merge<<<1, 64, 0, s->C.stream>>>(scratch, nb, result);
+  OX_HIP(hipGetLastError());
+  double r[ox::DIAG_NQ + 1];
+  OX_HIP(hipMemcpyAsync(r, result, sizeof(r), hipMemcpyDeviceToHost, s->C.stream));
+  OX_HIP(hipStreamSynchronize(s->C.stream));
+  const double ncell = (double)g.ix * (double)g.iy * (double)g.iz;
+  out->ke = r[0] / ncell;
+  out->entropy = r[1] / ncell;
+  out->enstrophy = r[2] / ncell;
+  out->max_wave_speed = r[ox::DIAG_NQ];
+  out->mass = r[3];
+  out->energy = r[4];
   return OX_OK;
 }
 

@@ -377,6 +377,7 @@ def run_form(spec, steps, warmup, profile):
     Q = cases.tgv_state(spec)
     with Solver(spec) as s:
         s.set_state(Q)
+        lseg = s.autotune()  # the library's segment-length tuner on this grid (the state is unchanged)
         if warmup:
             s.step(warmup)
         s.set_profiling(profile)
@@ -388,7 +389,7 @@ def run_form(spec, steps, warmup, profile):
         kt = s.kernel_times() if profile else {}
         diag = s.diagnostics()
         ok, _, _ = s.check_state()
-    return {"wall_s": wall, "dev_ms": dev_ms, "kernels": kt, "diag": diag, "ok": ok}
+    return {"wall_s": wall, "dev_ms": dev_ms, "kernels": kt, "diag": diag, "ok": ok, "lseg": lseg}
 
 
 def history_leg(n, nsteps, every):
@@ -411,6 +412,7 @@ def history_leg(n, nsteps, every):
         t0 = time.perf_counter()
         with Solver(spec) as s:
             s.init_state()
+            s.autotune()
             d = s.diagnostics()
             rows.append((0, d["ke"], d["enstrophy"]))
             for k in range(every, nsteps + 1, every):
@@ -526,7 +528,8 @@ def main():
         "config": {"workload": f"supersonic NS TGV {args.n}^3 periodic, Ma 1.25, Re 1600, SLAU ORDER 6 HR-SLAU2 "
                                "hybrid-TVD (threshold 0.4), central-6 viscous, TVD-RK3, CFL 0.03",
                    "interior": [args.n] * 3, "extents": [spec.nx, spec.ny, spec.nz], "stages_per_step": 3,
-                   "form": "FP64 plain (bit-identical to the CPU oracle)"},
+                   "form": "FP64 plain (bit-identical to the CPU oracle)",
+                   "segment_lengths_yz": list(r64["lseg"])},
         "fp32_bulk": {"value": round(value32, 1), "ms_per_step": round(r32["wall_s"] * 1e3 / args.steps, 3),
                       "speedup_vs_fp64": round(value32 / value, 3),
                       "ke_rel_err_vs_fp64": abs(ke32 - ke64) / abs(ke64)},

@@ -207,9 +207,9 @@ int build_plan(ox_batch* b) {
         if (d == 0) {
           nbx = cdivu(g.ix, G.ops.xt); nby = cdivu(g.iy, ox::XROWS); nbz = g.iz;
         } else if (d == 1) {
-          nbx = cdivu(g.ix, 64); nby = cdivu(g.iz, 4); nbz = cdivu(g.iy, ls[0]);
+          nbx = cdivu(g.ix, ox::LINE_X); nby = cdivu(g.iz, ox::LINE_Y); nbz = cdivu(g.iy, ls[0]);
         } else {
-          nbx = cdivu(g.ix, 64); nby = cdivu(g.iy, 4); nbz = cdivu(g.iz, ls[1]);
+          nbx = cdivu(g.ix, ox::LINE_X); nby = cdivu(g.iy, ox::LINE_Y); nbz = cdivu(g.iz, ls[1]);
         }
         m.start = nt[d];
         m.nbx = (int)nbx;

@@ -788,6 +788,14 @@ __device__ __forceinline__ void fetch_state(const S* __restrict__ Racc, const St
 // the stage epilogue.
 // ---------------------------------------------------------------------------
 constexpr int LSEG_DEFAULT = 64;
+// Line-sweep workgroup: LINE_X consecutive x columns by LINE_Y rows (z rows
+// for the y sweep, y rows for the z sweep), 256 threads.
+#ifndef OX_LINE_X
+#define OX_LINE_X 64
+#endif
+constexpr int LINE_X = OX_LINE_X;
+constexpr int LINE_Y = 256 / LINE_X;
+static_assert(LINE_X % 64 == 0 && LINE_X * LINE_Y == 256, "line-sweep workgroup of whole wavefronts");
 
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO, int DIR, int EP>
 __device__ __forceinline__ void sweep_line_tile(const SweepArgs<R, S>& a, int bx, int by, int bz) {
@@ -802,8 +810,8 @@ __device__ __forceinline__ void sweep_line_tile(const SweepArgs<R, S>& a, int bx
   const double cd = a.cd;
   using St = Sten<ORDER, VISC, VO>;
   using CD = CellData<R, DIR, ORDER, SCHEME, VISC, VO>;
-  const int i = g.h + bx * 64 + threadIdx.x;
-  const int t = g.h + by * 4 + threadIdx.y;  // k for DIR 1, j for DIR 2
+  const int i = g.h + bx * LINE_X + threadIdx.x;
+  const int t = g.h + by * LINE_Y + threadIdx.y;  // k for DIR 1, j for DIR 2
   const int s0 = g.h + bz * a.lseg;
   const int nt = DIR == 1 ? g.iz : g.iy;
   const int nd = DIR == 1 ? g.iy : g.iz;

@@ -26,8 +26,13 @@ template <class S> Prim<S> prim_set(const Core& C, int set) {
 template <int ORDER, int VISC, int VO> dim3 grid_x(const Geom& g) {
   return dim3(cdiv(g.ix, xtile<ORDER, VISC, VO>()), cdiv(g.iy, XROWS), g.iz);
 }
-inline dim3 grid_y(const Geom& g, int lseg) { return dim3(cdiv(g.ix, 64), cdiv(g.iz, 4), cdiv(g.iy, lseg)); }
-inline dim3 grid_z(const Geom& g, int lseg) { return dim3(cdiv(g.ix, 64), cdiv(g.iy, 4), cdiv(g.iz, lseg)); }
+inline dim3 grid_y(const Geom& g, int lseg) {
+  return dim3(cdiv(g.ix, LINE_X), cdiv(g.iz, LINE_Y), cdiv(g.iy, lseg));
+}
+inline dim3 grid_z(const Geom& g, int lseg) {
+  return dim3(cdiv(g.ix, LINE_X), cdiv(g.iy, LINE_Y), cdiv(g.iz, lseg));
+}
+inline dim3 line_block() { return dim3(LINE_X, LINE_Y, 1); }
 
 // What precedes the sweeps of a stage on one case: primitives when they do not
 // describe Qsrc, and for LES the SGS viscosity of the stage.
@@ -85,10 +90,10 @@ void run_stage(Core& C, const double* Qsrc, const StageArgs& sa, double cx, doub
   make_sweep_args<R, S>(C, sa, cx, cy, cz, a);
   k_sweep_x<R, S, ORDER, SCHEME, VISC, VO><<<grid_x<ORDER, VISC, VO>(g), blk, 0, C.stream>>>(a[0]);
   prof_mark(C, "sweep_x");
-  k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 1, EP_RHS><<<grid_y(g, a[1].lseg), blk, 0, C.stream>>>(a[1]);
+  k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 1, EP_RHS><<<grid_y(g, a[1].lseg), line_block(), 0, C.stream>>>(a[1]);
   prof_mark(C, "sweep_y");
   const dim3 gz = grid_z(g, a[2].lseg);
-#define OX_Z(E) k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 2, E><<<gz, blk, 0, C.stream>>>(a[2])
+#define OX_Z(E) k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 2, E><<<gz, line_block(), 0, C.stream>>>(a[2])
   switch (sa.ep) {
     case EP_RK3_S1: OX_Z(EP_RK3_S1); break;
     case EP_RK3_S23: OX_Z(EP_RK3_S23); break;
@@ -109,10 +114,10 @@ void run_stage_batch(const BatchGroupLaunch& L) {
   k_sweep_x_batch<R, S, ORDER, SCHEME, VISC, VO>
       <<<L.grid[0], blk, 0, L.stream>>>((const A*)L.args[0], L.map[0], L.ncase, L.ntiles[0]);
   k_sweep_line_batch<R, S, ORDER, SCHEME, VISC, VO, 1, EP_RHS>
-      <<<L.grid[1], blk, 0, L.stream>>>((const A*)L.args[1], L.map[1], L.ncase, L.ntiles[1]);
+      <<<L.grid[1], line_block(), 0, L.stream>>>((const A*)L.args[1], L.map[1], L.ncase, L.ntiles[1]);
 #define OX_ZB(E)                                         \
   k_sweep_line_batch<R, S, ORDER, SCHEME, VISC, VO, 2, E> \
-      <<<L.grid[2], blk, 0, L.stream>>>((const A*)L.args[2], L.map[2], L.ncase, L.ntiles[2])
+      <<<L.grid[2], line_block(), 0, L.stream>>>((const A*)L.args[2], L.map[2], L.ncase, L.ntiles[2])
   switch (L.ep) {
     case EP_RK3_S1: OX_ZB(EP_RK3_S1); break;
     case EP_RK3_S23: OX_ZB(EP_RK3_S23); break;

@@ -39,6 +39,20 @@ def _interior(spec):
     return (slice(h, spec.nz - h), slice(h, spec.ny - h), slice(None), slice(h, spec.nx - h))
 
 
+def _assert_same(got, want, what):
+    """Exact equality; on a mismatch, report where it is (per variable and
+    per index along each axis) before failing."""
+    bad = got != want
+    if bad.any():
+        k, j, l, i = np.nonzero(bad)
+        print(f"{what}: {int(bad.sum())} of {bad.size} differ; per variable {np.bincount(l, minlength=5).tolist()}")
+        for name, idx, n in (("x", i, got.shape[3]), ("y", j, got.shape[1]), ("z", k, got.shape[0])):
+            print(f"  along {name}: {np.bincount(idx, minlength=n).tolist()}")
+        d = np.abs(got - want)[bad] / np.maximum(np.abs(want)[bad], 1e-300)
+        print(f"  relative difference max {d.max():.3e} median {np.median(d):.3e}", flush=True)
+    np.testing.assert_array_equal(got, want, err_msg=what)
+
+
 def _seam_spec(name):
     return _spec(*SCHEMES[name], n=SEAM_N, Ma=1.25)
 
@@ -98,8 +112,8 @@ def test_seam_les_dhit_bitwise():
         s.step(2)
         QJg = s.get_qj()
     sl = _interior(spec)
-    np.testing.assert_array_equal(Rg[sl], Ro[sl])
-    np.testing.assert_array_equal(QJg, o.get_qj())
+    _assert_same(Rg[sl], Ro[sl], "rhs")
+    _assert_same(QJg, o.get_qj(), "state after 2 steps")
 
 
 def test_segment_setter_rejects_bad_lengths():

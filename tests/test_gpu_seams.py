@@ -97,11 +97,46 @@ def test_seam_segment_lengths_bitwise(lseg):
     np.testing.assert_array_equal(QJg, o.get_qj())
 
 
-def test_seam_les_dhit_bitwise():
+def _numpy_phase_field(spec):
+    """A second turbulent field of the same spectrum: the mode phases drawn
+    from numpy's generator (seeded) instead of the counter-based draw, so the
+    seams meet two independent realisations."""
+    n = spec.nx - spec.order
+    rng = np.random.default_rng(spec.seed)
+    k1 = np.fft.fftfreq(n, d=1.0 / n)
+    kr = np.fft.rfftfreq(n, d=1.0 / n)
+    KZ, KY, KX = np.meshgrid(k1, k1, kr, indexing="ij")
+    kmag = np.sqrt(KX * KX + KY * KY + KZ * KZ)
+    k12 = np.sqrt(KX * KX + KY * KY)
+    nyq = (np.abs(KX) == n // 2) | (np.abs(KY) == n // 2) | (np.abs(KZ) == n // 2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        E = kmag ** 4 * np.exp(-2.0 * (kmag / 4.0) ** 2)
+        amp = np.where((kmag > 0) & ~nyq, np.sqrt(E / (2.0 * np.pi * kmag * kmag)), 0.0)
+        ph = rng.uniform(0.0, 2.0 * np.pi, (3,) + kmag.shape)
+        a = amp * np.exp(1j * ph[0]) * np.cos(ph[2])
+        b = amp * np.exp(1j * ph[1]) * np.sin(ph[2])
+        s12 = np.where(k12 > 0, k12, 1.0)
+        sk = np.where(kmag > 0, kmag, 1.0)
+        ux = np.where(k12 > 0, (KY / s12) * a + (KX / s12) * (KZ / sk) * b, a)
+        uy = np.where(k12 > 0, -(KX / s12) * a + (KY / s12) * (KZ / sk) * b, 0.0)
+        uz = np.where(k12 > 0, -(k12 / sk) * b, 0.0)
+    u = np.stack([np.fft.irfftn(c, s=(n, n, n)) for c in (ux, uy, uz)])
+    u *= spec.V0 / np.sqrt(np.mean(np.sum(u * u, axis=0)))
+    h = spec.order // 2
+    Q = np.zeros(spec.state_shape())
+    Q[h:-h, h:-h, 0, h:-h] = spec.RHO0
+    for c in range(3):
+        Q[h:-h, h:-h, 1 + c, h:-h] = spec.RHO0 * u[c]
+    Q[h:-h, h:-h, 4, h:-h] = spec.p0 / (spec.gamma - 1.0) + 0.5 * spec.RHO0 * np.sum(u * u, axis=0)
+    return cases.periodic_fill(Q, spec.order)
+
+
+@pytest.mark.parametrize("field", ["counter-phases", "numpy-phases"])
+def test_seam_les_dhit_bitwise(field):
     """Config 3's scheme (Hybrid KEEP/SLAU ORDER 6, SMS LES, RK4) on its
     turbulent field at 124^3: three x tiles and two segments per line sweep."""
     spec = cases.dhit_spec(124, "fp64", seed=5)
-    Q = cases.initial_state(spec)
+    Q = cases.initial_state(spec) if field == "counter-phases" else _numpy_phase_field(spec)
     o = Oracle(spec)
     o.set_state(Q)
     Ro = o.rhs()

@@ -222,7 +222,9 @@ int ox_set_segments(ox_solver* s, int32_t lseg_y, int32_t lseg_z);
  * Batch engine: many cases resident on one GPU, advanced together.  Cases
  * that share a kernel specialisation and RK scheme form a group; one RK stage
  * of a group is one persistent launch per sweep over the tiles of all its
- * cases, and one step of the whole batch is replayed from a captured HIP
+ * cases (LES groups first run the SGS viscosity, calc_les.f90:96-125, and
+ * its ghost copy, set_bc_common.f90:296-335, as persistent launches over the
+ * same cases), and one step of the whole batch is replayed from a captured HIP
  * graph while the set of cases does not change.  Cases join (ox_batch_admit)
  * and leave (when their step count is reached) between steps.  The reference
  * has no counterpart: it runs one case per `mpirun` (ouxsbli/case.py:180-201).

@@ -70,9 +70,11 @@ struct BCase {
 };
 
 // Cases sharing a kernel specialisation.  3-D groups launch three sweeps per
-// stage; 2-D groups one fused stage kernel (launch2[stage]).
+// stage (LES groups first the SGS viscosity and its two ghost-copy passes);
+// 2-D groups one fused stage kernel (launch2[stage]).
 struct BGroup {
   int dim, prec, key, rk;
+  bool les;
   ox::BatchOps ops;
   ox2::Launch2 launch2[3];
   std::vector<BCase*> members;
@@ -171,6 +173,7 @@ int build_plan(ox_batch* b) {
       G.prec = prec;
       G.key = key;
       G.rk = rk;
+      G.les = dim == 3 && c->s->C.cfg.visc == OX_VISC_LES;
       if (dim == 3) {
         const ox::Core& C = c->s->C;
         const bool ok = prec == OX_PREC_FP64 ? ox::find_batch_fp64(C.cfg.order, C.cfg.scheme, C.cfg.visc, C.vorder, &G.ops)
@@ -185,17 +188,21 @@ int build_plan(ox_batch* b) {
     c->group = gi;
     b->groups[gi].members.push_back(c);
   }
-  // tile maps: per 3-D group, per direction, one TileMap per member; 2-D
-  // groups count their tiles (the case lookup reads the descriptors)
+  // tile maps: per 3-D group, per direction, one TileMap per member (LES
+  // groups: three more, for the SGS viscosity and the two ghost-copy
+  // passes); 2-D groups count their tiles (the case lookup reads the
+  // descriptors)
   std::vector<ox::TileMap> maps;
-  std::vector<size_t> map_off;  // per group: index of its x map (y at +n, z at +2n)
-  std::vector<std::array<int, 3>> ntiles;
+  std::vector<size_t> map_off;  // per group: index of its x map (y at +n, z at +2n, LES at +3n..+5n)
+  std::vector<std::array<int, 3>> ntiles, ntiles_les;
   for (BGroup& G : b->groups) {
     map_off.push_back(maps.size());
     std::array<int, 3> nt{0, 0, 0};
+    std::array<int, 3> nl{0, 0, 0};
     if (G.dim == 2) {
       for (BCase* c : G.members) nt[0] += c->s2->ntiles;
       ntiles.push_back(nt);
+      ntiles_les.push_back(nl);
       continue;
     }
     for (int d = 0; d < 3; ++d) {
@@ -220,6 +227,27 @@ int build_plan(ox_batch* b) {
       }
     }
     ntiles.push_back(nt);
+    if (G.les) {
+      for (int d = 0; d < 3; ++d) {
+        for (BCase* c : G.members) {
+          const ox::Geom& g = c->s->C.g;
+          ox::TileMap m{};
+          if (d == 0) {  // 64 x 4 cells of one plane, as k_les_mut's blocks
+            m.nbx = (int)cdivu(g.nx - 2, 64);
+            m.nby = (int)cdivu(g.ny - 2, 4);
+            m.nbz = g.nz - 2;
+          } else {  // 256 items of a ghost-copy pass
+            m.nbx = (int)cdivu(ox::les_bc_items(g, d - 1), 256);
+            m.nby = 1;
+            m.nbz = 1;
+          }
+          m.start = nl[d];
+          nl[d] += m.nbx * m.nby * m.nbz;
+          maps.push_back(m);
+        }
+      }
+    }
+    ntiles_les.push_back(nl);
   }
   if (!maps.empty()) {
     int rc = ensure_device((char**)&b->d_map, &b->d_map_cap, maps.size() * sizeof(ox::TileMap));
@@ -246,6 +274,11 @@ int build_plan(ox_batch* b) {
           L.map[d] = b->d_map + map_off[gi] + (size_t)d * n;
           L.ntiles[d] = ntiles[gi][d];
           L.grid[d] = std::max(1, std::min(ntiles[gi][d], kMaxGrid));
+          if (G.les) {
+            L.map_les[d] = b->d_map + map_off[gi] + (size_t)(3 + d) * n;
+            L.ntiles_les[d] = ntiles_les[gi][d];
+            L.grid_les[d] = std::max(1, std::min(ntiles_les[gi][d], kMaxGrid));
+          }
         }
       } else {
         L.ntiles[0] = ntiles[gi][0];
@@ -406,7 +439,6 @@ template <class Init>
 int admit3d(ox_batch* b, const ox_config* cfg, Init init, int32_t nsteps, int64_t* id) {
   if (nsteps < 1) return ox_fail(OX_EINVAL, "nsteps must be >= 1");
   if (cfg->device != b->cfg.device) return ox_fail(OX_EINVAL, "case device differs from the batch device");
-  if (cfg->visc == OX_VISC_LES) return ox_fail(OX_EUNSUPPORTED, "LES cases are not batched (use ox_step)");
   const size_t need = solver_footprint(*cfg);
   if (need > b->cfg.hbm_budget)
     return ox_fail(OX_EINVAL, "case footprint " + std::to_string(need) + " B exceeds the whole HBM budget " +

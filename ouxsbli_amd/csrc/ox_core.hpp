@@ -78,6 +78,10 @@ struct BatchGroupLaunch {
   int ntiles[3];
   int grid[3];             // persistent grid sizes
   int ep;                  // RK epilogue of the stage
+  // LES groups: the SGS viscosity and its two ghost-copy passes
+  const TileMap* map_les[3];
+  int ntiles_les[3];
+  int grid_les[3];
 };
 
 struct BatchOps {

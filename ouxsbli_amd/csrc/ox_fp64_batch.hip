@@ -9,6 +9,7 @@ bool find_batch_fp64(int order, int scheme, int visc, int vorder, BatchOps* ops)
   using S_ = double;
   switch (stage_key(order, scheme, visc, vorder)) {
     OX_FOR_EACH_STAGE(OX_BATCH_CASE)
+    OX_FOR_EACH_LES_STAGE(OX_BATCH_CASE)
     default: return false;
   }
 }

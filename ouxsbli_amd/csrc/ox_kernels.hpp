@@ -123,9 +123,9 @@ __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ QJ, Pri
 // ---------------------------------------------------------------------------
 template <class S, class R>
 __device__ __forceinline__ R tfilt(const S* __restrict__ f, int m, int sy, int sz) {
-  const R c0 = R(f[m]);
-  return c0 + R(0.25) * (R(f[m - 1]) - R(2) * c0 + R(f[m + 1]) + R(f[m - sy]) - R(2) * c0 + R(f[m + sy]) +
-                         R(f[m - sz]) - R(2) * c0 + R(f[m + sz]));
+  const R c0 = R(G(f)[m]);
+  return c0 + R(0.25) * (R(G(f)[m - 1]) - R(2) * c0 + R(G(f)[m + 1]) + R(G(f)[m - sy]) - R(2) * c0 +
+                         R(G(f)[m + sy]) + R(G(f)[m - sz]) - R(2) * c0 + R(G(f)[m + sz]));
 }
 
 template <class R>
@@ -142,12 +142,10 @@ __device__ __forceinline__ void grad9(R um, R up, R vm, R vp, R wm, R wp, R dx, 
   g[8] = R(0.5) * (-wz[0] + wz[1]) * dz;
 }
 
+// nu_t and qc2 of cell (i, j, k), 1 <= i <= nx-2, 1 <= j <= ny-2, 1 <= k <= nz-2.
 template <class S, class R>
-__global__ __launch_bounds__(256) void k_les_mut(Prim<S> P, Geom g, PhysC<R> c) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 1 + blockIdx.z;
-  if (i > g.nx - 2 || j > g.ny - 2) return;
+__device__ __forceinline__ void les_mut_cell(const Prim<S>& P, const Geom& g, const PhysC<R>& c, int i, int j,
+                                             int k) {
   const int sy = g.nx, sz = g.nx * g.ny;
   const int m = cidx(g, i, j, k);
   const bool filt = (i >= 2 && i <= g.nx - 3 && j >= 2 && j <= g.ny - 3 && k >= 2 && k <= g.nz - 3);
@@ -162,7 +160,7 @@ __global__ __launch_bounds__(256) void k_les_mut(Prim<S> P, Geom g, PhysC<R> c) 
   for (int f = 0; f < 3; ++f)
 #pragma unroll
     for (int t = 0; t < 7; ++t) {
-      raw[f][t] = R(U[f][m + off[t]]);
+      raw[f][t] = R(G(U[f])[m + off[t]]);
       const R ft = tfilt<S, R>(U[f], filt ? m + off[t] : m, sy, sz);
       fil[f][t] = filt ? ft : raw[f][t];
     }
@@ -203,13 +201,23 @@ __global__ __launch_bounds__(256) void k_les_mut(Prim<S> P, Geom g, PhysC<R> c) 
   G(P.mut)[m] = (S)(R(G(P.rho)[m]) * nut);
 }
 
+template <class S, class R>
+__global__ __launch_bounds__(256) void k_les_mut(Prim<S> P, Geom g, PhysC<R> c) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.nx - 2 || j > g.ny - 2) return;
+  les_mut_cell<S, R>(P, g, c, i, j, k);
+}
+
 // set_bc_mut_common (3D_solver/src/set_bc_common.f90:296-335): pass 0 = x
 // faces, y faces and xy corners over k = Fortran 2..nz-1; pass 1 = z planes.
+// Item t of a pass (items past the pass's count do nothing); pass 1 reads
+// what pass 0 wrote, so the two run as separate launches.
 template <class S>
-__global__ __launch_bounds__(256) void k_les_bc(Prim<S> P, Geom g, int pass) {
-  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void les_bc_item(const Prim<S>& P, const Geom& g, int pass, long long t) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  S* A[2] = {P.mut, P.qc2};
+  __attribute__((address_space(1))) S* A[2] = {G(P.mut), G(P.qc2)};
   if (pass == 0) {
     const int nk = nz - 2;
     const long long nxf = (long long)(ny - 2) * nk, nyf = (long long)(nx - 2) * nk;
@@ -244,6 +252,17 @@ __global__ __launch_bounds__(256) void k_les_bc(Prim<S> P, Geom g, int pass) {
       }
     }
   }
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void k_les_bc(Prim<S> P, Geom g, int pass) {
+  les_bc_item<S>(P, g, pass, (long long)blockIdx.x * 256 + threadIdx.x);
+}
+
+// Items of the two passes of les_bc_item over a grid.
+inline long long les_bc_items(const Geom& g, int pass) {
+  return pass == 0 ? (long long)(g.ny - 2) * (g.nz - 2) + (long long)(g.nx - 2) * (g.nz - 2) + (g.nz - 2)
+                   : (long long)g.nx * g.ny;
 }
 
 // ---------------------------------------------------------------------------
@@ -948,6 +967,36 @@ __global__ __launch_bounds__(256) void k_sweep_line_batch(const SweepArgs<R, S>*
     const int local = t - map[ci].start, nbx = map[ci].nbx, nby = map[ci].nby;
     sweep_line_tile<R, S, ORDER, SCHEME, VISC, VO, DIR, EP>(cases[ci], local % nbx, (local / nbx) % nby,
                                                             local / (nbx * nby));
+  }
+}
+
+// Persistent batched SGS viscosity of LES cases over the tiles of several
+// cases (tile = 64 x 4 cells of one plane, the single-case launch's block),
+// reading each case's x-sweep descriptor: the primitive set the stage reads,
+// the geometry and the constants.  Blocks are 64 x 4.
+template <class R, class S>
+__global__ __launch_bounds__(256) void k_les_mut_batch(const SweepArgs<R, S>* __restrict__ cases,
+                                                       const TileMap* __restrict__ map, int ncase, int ntiles) {
+  for (int t = xcd_tile(blockIdx.x, gridDim.x); t < ntiles; t += gridDim.x) {
+    const int ci = find_case(map, ncase, t);
+    const int local = t - map[ci].start, nbx = map[ci].nbx, nby = map[ci].nby;
+    const SweepArgs<R, S>& a = cases[ci];
+    const int i = 1 + (local % nbx) * 64 + (int)threadIdx.x;
+    const int j = 1 + ((local / nbx) % nby) * 4 + (int)threadIdx.y;
+    const int k = 1 + local / (nbx * nby);
+    if (i <= a.g.nx - 2 && j <= a.g.ny - 2) les_mut_cell<S, R>(a.P, a.g, a.c, i, j, k);
+  }
+}
+
+// Persistent batched ghost copy of nu_t and qc2, one pass (tile = 256 items).
+// Blocks are 256 x 1.
+template <class R, class S>
+__global__ __launch_bounds__(256) void k_les_bc_batch(const SweepArgs<R, S>* __restrict__ cases,
+                                                      const TileMap* __restrict__ map, int ncase, int ntiles,
+                                                      int pass) {
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int ci = find_case(map, ncase, t);
+    les_bc_item<S>(cases[ci].P, cases[ci].g, pass, (long long)(t - map[ci].start) * 256 + threadIdx.x);
   }
 }
 

@@ -50,9 +50,8 @@ void prepare_stage(Core& C, const double* Qsrc) {
   if constexpr (VISC == 2) {
     const PhysC<R> cr = make_physc<R>(C.hc);
     k_les_mut<S, R><<<dim3(cdiv(g.nx - 2, 64), cdiv(g.ny - 2, 4), g.nz - 2), blk, 0, C.stream>>>(P, g, cr);
-    const long long n0 = (long long)(g.ny - 2) * (g.nz - 2) + (long long)(g.nx - 2) * (g.nz - 2) + (g.nz - 2);
-    k_les_bc<S><<<(unsigned)((n0 + 255) / 256), 256, 0, C.stream>>>(P, g, 0);
-    k_les_bc<S><<<(unsigned)(((long long)g.nx * g.ny + 255) / 256), 256, 0, C.stream>>>(P, g, 1);
+    k_les_bc<S><<<(unsigned)((les_bc_items(g, 0) + 255) / 256), 256, 0, C.stream>>>(P, g, 0);
+    k_les_bc<S><<<(unsigned)((les_bc_items(g, 1) + 255) / 256), 256, 0, C.stream>>>(P, g, 1);
     prof_mark(C, "les_mut");
   }
 }
@@ -106,11 +105,19 @@ void run_stage(Core& C, const double* Qsrc, const StageArgs& sa, double cx, doub
   finish_stage(C, sa);
 }
 
-// One stage of a group of cases: three persistent launches.
+// One stage of a group of cases: three persistent launches (LES: the SGS
+// viscosity and its two ghost-copy passes first, as in prepare_stage).
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
 void run_stage_batch(const BatchGroupLaunch& L) {
   const dim3 blk(64, 4, 1);
   using A = SweepArgs<R, S>;
+  if constexpr (VISC == 2) {
+    k_les_mut_batch<R, S><<<L.grid_les[0], blk, 0, L.stream>>>((const A*)L.args[0], L.map_les[0], L.ncase,
+                                                                L.ntiles_les[0]);
+    for (int pass = 0; pass < 2; ++pass)
+      k_les_bc_batch<R, S><<<L.grid_les[1 + pass], 256, 0, L.stream>>>((const A*)L.args[0], L.map_les[1 + pass],
+                                                                        L.ncase, L.ntiles_les[1 + pass], pass);
+  }
   k_sweep_x_batch<R, S, ORDER, SCHEME, VISC, VO>
       <<<L.grid[0], blk, 0, L.stream>>>((const A*)L.args[0], L.map[0], L.ncase, L.ntiles[0]);
   k_sweep_line_batch<R, S, ORDER, SCHEME, VISC, VO, 1, EP_RHS>
@@ -172,7 +179,7 @@ template <class R, class S, int ORDER, int SCHEME, int VISC, int VO> BatchOps ba
   X(6, KEEP, 1, 5)           \
   X(6, SLAU, 1, 5)           \
   X(6, HYBRID, 1, 5)
-// ... and with the LES model (single-case launches only).  VO 5 is ME4Base.
+// ... and with the LES model.  VO 5 is ME4Base.
 #define OX_FOR_EACH_LES_STAGE(X) \
   X(4, KEEP, 2, 4)               \
   X(6, KEEP, 2, 6)               \

@@ -27,7 +27,7 @@ import numpy as np
 
 from ._native import OxError
 from .batch import Batch, Busy
-from .cases import CaseSpec, initial_state, khi_spec
+from .cases import CaseSpec, dhit_spec, initial_state, khi_spec
 
 
 @dataclasses.dataclass
@@ -70,6 +70,26 @@ def khi_stream(n_requests: int, rate_hz: float = 20.0, seed: int = 20261207, n_m
         with concurrent.futures.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
             for r, Q in zip(reqs, ex.map(lambda q: initial_state(q.spec), reqs)):
                 r.Q0 = Q
+    return reqs
+
+
+def dhit_stream(n_requests: int, rate_hz: float = 4.0, seed: int = 20261208, n_min: int = 64, n_max: int = 192,
+                multiple: int = 32, nsteps: int = 50, precision: str = "fp32_bulk",
+                with_states: bool = False) -> List[Request]:
+    """An LES request stream: decaying isotropic turbulence (config 3's
+    scheme: Hybrid KEEP/SLAU ORDER 6, SMS model, RK4) on interiors uniform
+    over the multiples of ``multiple`` in [n_min, n_max], each with its own
+    spectral seed, exponential inter-arrival times (mean 1/rate_hz).  By
+    default the requests carry parameters only and the engine synthesises
+    each initial field on the device."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for r in range(n_requests):
+        t += float(rng.exponential(1.0 / rate_hz))
+        n = int(rng.integers(n_min // multiple, n_max // multiple + 1)) * multiple
+        spec = dhit_spec(n, precision, seed=seed + 1 + r)
+        reqs.append(Request(r, spec, nsteps, t, Q0=initial_state(spec) if with_states else None))
     return reqs
 
 
@@ -216,24 +236,30 @@ def serve(requests: List[Request], hbm_budget: int, max_cases: int = 64, use_gra
 
 
 def main(argv=None) -> int:
-    """Runs a Kelvin-Helmholtz request stream and prints one JSON line with
-    the latency percentiles, throughput and engine statistics."""
+    """Runs a request stream (Kelvin-Helmholtz, or LES isotropic turbulence)
+    and prints one JSON line with the latency percentiles, throughput and
+    engine statistics."""
     import argparse
     import json
     ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--stream", default="khi", choices=["khi", "dhit"])
     ap.add_argument("--requests", type=int, default=24)
-    ap.add_argument("--rate", type=float, default=20.0, help="mean arrivals per second")
+    ap.add_argument("--rate", type=float, default=None, help="mean arrivals per second (khi 20, dhit 4)")
     ap.add_argument("--n-min", type=int, default=64)
-    ap.add_argument("--n-max", type=int, default=256)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--n-max", type=int, default=None, help="largest interior (khi 256, dhit 192)")
+    ap.add_argument("--steps", type=int, default=None, help="steps per request (khi 200, dhit 50)")
     ap.add_argument("--precision", default="fp32_bulk", choices=["fp64", "fp32_bulk"])
     ap.add_argument("--budget-gib", type=float, default=64.0)
     ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--seed", type=int, default=20261207)
+    ap.add_argument("--seed", type=int, default=None)
     a = ap.parse_args(argv)
-    reqs = khi_stream(a.requests, rate_hz=a.rate, seed=a.seed, n_min=a.n_min, n_max=a.n_max, nsteps=a.steps,
-                      precision=a.precision)
+    if a.stream == "khi":
+        reqs = khi_stream(a.requests, rate_hz=a.rate or 20.0, seed=a.seed or 20261207, n_min=a.n_min,
+                          n_max=a.n_max or 256, nsteps=a.steps or 200, precision=a.precision, with_states=False)
+    else:
+        reqs = dhit_stream(a.requests, rate_hz=a.rate or 4.0, seed=a.seed or 20261208, n_min=a.n_min,
+                           n_max=a.n_max or 192, nsteps=a.steps or 50, precision=a.precision)
     rep = serve(reqs, hbm_budget=int(a.budget_gib * (1 << 30)), max_cases=a.slots, use_graph=not a.no_graph,
                 keep_results=False)
     print(json.dumps({**rep.summary(), "engine": rep.batch_stats}), flush=True)

@@ -115,6 +115,85 @@ def test_batched_fp64_equals_oracle(n):
     np.testing.assert_array_equal(Qg, o.get_state())
 
 
+def _les_mix(precision):
+    # (spec, nsteps, admitted before batch step #): LES cases of two
+    # specialisations (ORDER 6 central-6 and ORDER 4 central-4) and sizes,
+    # joining and leaving among non-LES cases
+    return [
+        (cases.dhit_spec(16, precision, seed=3), 3, 0),
+        (cases.dhit_spec(24, precision, seed=4), 2, 1),
+        (cases.dhit_spec(20, precision, seed=5, order=4), 3, 0),
+        (cases.khi_spec(24, 6, "SLAU", precision, seed=6), 4, 0),
+        (cases.tgv_spec(20, 6, Ma=1.25, cfl=0.3, cfl_mode="acoustic", scheme="SLAU", visc="NS", visc_order=6, rk=3,
+                        tvd="hybrid", precision=precision), 2, 2),
+    ]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_bulk"])
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_les_alone_equals_in_mix(precision, use_graph):
+    """LES cases in the batch (persistent SGS-viscosity and ghost-copy
+    launches ahead of the sweeps, replayed from the step graphs) end
+    bit-identical to the same cases advanced alone with ox_step."""
+    mix = _les_mix(precision)
+    states = [cases.initial_state(s) for s, _, _ in mix]
+    ref = [_alone(s, Q, n) for (s, n, _), Q in zip(mix, states)]
+    out = {}
+    with Batch(hbm_budget=8 * GIB, max_cases=8, use_graph=use_graph, autotune=False) as b:
+        ids = {}
+        step = 0
+        while len(out) < len(mix):
+            for k, (s, n, at) in enumerate(mix):
+                if at == step and k not in ids.values():
+                    ids[b.admit(s, states[k], n)] = k
+            b.step()
+            step += 1
+            for cid in b.done():
+                out[ids[cid]] = b.retire(cid)
+            assert step < 20
+    for k in range(len(mix)):
+        np.testing.assert_array_equal(out[k], ref[k], err_msg=f"case {k}")
+
+
+def test_batched_les_fp64_equals_oracle():
+    """A batched FP64 LES case (config 3's scheme on its turbulent field)
+    equals the oracle bit for bit; 72^3 gives two segments per line sweep
+    at the default segment length and two tiles of the SGS launch along x."""
+    spec = cases.dhit_spec(72, "fp64", seed=6)
+    Q = cases.initial_state(spec)
+    o = Oracle(spec)
+    o.set_state(Q)
+    o.step(2)
+    with Batch(hbm_budget=8 * GIB, max_cases=4, autotune=False) as b:
+        a = b.admit(spec, Q, 2)
+        other = cases.dhit_spec(16, "fp64", seed=7)
+        b.admit(other, cases.initial_state(other), 3)
+        while a not in b.done():
+            b.step()
+        Qg = b.retire(a)
+    np.testing.assert_array_equal(Qg, o.get_state())
+
+
+def _alone_device_ic(spec, n):
+    with Solver(spec) as s:
+        s.init_state()
+        s.step(n)
+        return s.get_state()
+
+
+def test_serve_les_stream_results_equal_alone():
+    """Served LES requests (parameters only: each initial field synthesised
+    on the device at admission) end bit-identical to the same requests run
+    alone from the same device-built field."""
+    reqs = serve.dhit_stream(4, rate_hz=100.0, seed=12, n_min=16, n_max=32, multiple=8, nsteps=3,
+                             precision="fp32_bulk")
+    assert all(r.Q0 is None for r in reqs)
+    rep = serve.serve(reqs, hbm_budget=4 * GIB, max_cases=3)
+    for r in rep.requests:
+        assert r.error is None and r.finished is not None
+        np.testing.assert_array_equal(r.result, _alone_device_ic(r.spec, r.nsteps), err_msg=f"request {r.rid}")
+
+
 def test_admission_by_footprint_and_slots():
     s1 = cases.khi_spec(32, 4, "KEEP", "fp64", seed=1)
     s2 = cases.khi_spec(24, 6, "SLAU", "fp64", seed=2)

@@ -104,6 +104,18 @@ def test_stream_is_seeded_and_in_range():
     assert abs(gaps.mean() - 0.05) < 0.01  # mean inter-arrival 1/rate
 
 
+def test_les_stream_is_seeded_parameters_only():
+    a = serve.dhit_stream(12, seed=5)
+    b = serve.dhit_stream(12, seed=5)
+    assert [(r.arrival, r.spec.nx, r.spec.seed) for r in a] == [(r.arrival, r.spec.nx, r.spec.seed) for r in b]
+    assert len({r.spec.seed for r in a}) == len(a)  # each request its own field
+    for r in a:
+        n = r.spec.nx - r.spec.order
+        assert 64 <= n <= 192 and n % 32 == 0
+        assert r.Q0 is None and r.spec.device_ic() is not None
+        assert (r.spec.visc, r.spec.scheme, r.spec.order, r.spec.rk) == ("LES", "Hybrid", 6, 4)
+
+
 def test_all_requests_complete_with_fifo_admission():
     clock = FakeClock()
     fb = FakeBatch(clock, budget=10 ** 9, slots=2)

@@ -371,6 +371,38 @@ def serving_leg(n_requests, fp64_value):
     return out
 
 
+def les_serving_leg(n_requests, nsteps=50):
+    """LES request stream through the batch engine: decaying isotropic
+    turbulence (Hybrid KEEP/SLAU ORDER 6, SMS model, central-6 viscous, RK4),
+    seeded Poisson arrivals at 4 req/s, interior grids 64^3..192^3
+    (multiples of 32), `nsteps` steps each, each request's turbulent field
+    synthesised on the device at admission from its own spectral seed.  Warm
+    engines (an earlier stream of the same shapes, 2 steps each), FP32-bulk
+    and FP64 on the same requests."""
+    from ouxsbli_amd import serve
+    from ouxsbli_amd.batch import Batch
+    seed = 20261208
+    out = {"workload": f"LES isotropic-turbulence stream: {n_requests} requests, Poisson 4 req/s, grids "
+                       f"64^3-192^3, Hybrid KEEP/SLAU ORDER 6, SMS SGS, central-6 viscous, RK4, {nsteps} steps "
+                       "each, fields synthesised on the device at admission, batched per stage (SGS viscosity "
+                       "included) with HIP graph replay",
+           "hbm_budget_GiB": SERVE_BUDGET >> 30, "slots": SERVE_SLOTS}
+    res = {}
+    for prec in ("fp32_bulk", "fp64"):
+        with Batch(SERVE_BUDGET, max_cases=SERVE_SLOTS, use_graph=True, autotune=True) as b:
+            serve.serve(serve.dhit_stream(n_requests, seed=seed + 1000, nsteps=2, precision=prec),
+                        hbm_budget=SERVE_BUDGET, max_cases=SERVE_SLOTS, keep_results=False, batch=b)
+            rep = serve.serve(serve.dhit_stream(n_requests, seed=seed, nsteps=nsteps, precision=prec),
+                              hbm_budget=SERVE_BUDGET, max_cases=SERVE_SLOTS, keep_results=False, batch=b)
+        res[prec] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in rep.summary().items()}
+    out["fp32_bulk"] = res["fp32_bulk"]
+    out["fp64"] = res["fp64"]
+    b32 = res["fp32_bulk"]["device_busy_Mcell_updates_per_s_stage"]
+    b64 = res["fp64"]["device_busy_Mcell_updates_per_s_stage"]
+    out["fp32_vs_fp64_same_requests_device_busy"] = round(b32 / b64, 3) if b32 and b64 else None
+    return out
+
+
 def run_form(spec, steps, warmup, profile):
     from ouxsbli_amd import cases
     from ouxsbli_amd.solver import Solver
@@ -460,6 +492,8 @@ def main():
                     help="run only the batched oblique-shock leg and print its JSON (for profiling)")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the Kelvin-Helmholtz serving stream (0: skip the serving leg)")
+    ap.add_argument("--les-serve-requests", type=int, default=12,
+                    help="requests of the LES serving stream (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the library holds each case on one GPU "
@@ -569,6 +603,8 @@ def main():
                                                        0.0 if args.no_cpu else 5.0)
     if args.serve_requests > 0:
         line["serving"] = serving_leg(args.serve_requests, value)
+    if args.les_serve_requests > 0:
+        line["les_serving"] = les_serving_leg(args.les_serve_requests)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -64,6 +64,19 @@ extern "C" {
 /* VISC_STENCIL (3D_solver/src/calc_visc_high_internal.f90.fypp:5-7) */
 #define OX_STENCIL_CENTRAL 0
 #define OX_STENCIL_ME4BASE 1
+/* Boundary kind of one direction (both ends), the ghost fill that the case's
+ * set_bc applies after every stage.  PERIODIC: the periodic image
+ * (set_bc_cyclic*, 3D_solver/src/set_bc_common.f90:101-273).  EXTRAPOLATE:
+ * every ghost layer takes the adjacent interior cell (zero gradient,
+ * 3D_solver/STZ/set.f90:113-138).  SLIP: mirror images about the boundary face
+ * with the normal momentum negated (slip wall, 3D_solver/STZ/set.f90:77-112).
+ * The faces of interior cells read only full-order stencils inside the ghost
+ * layers, so the interior result equals the reference's BC_X/Y/Z=True flux
+ * kernels (calc_*_kernel.f90.fypp) followed by that ghost fill. */
+#define OX_BC_PERIODIC    0
+#define OX_BC_EXTRAPOLATE 1
+#define OX_BC_SLIP        2
+
 /* Arithmetic of the RHS. FP64: everything FP64 (bit-identical to the FP64
  * CPU transcription). FP32_BULK: flux/stencil arithmetic in FP32, conserved
  * state, RK accumulation and primitive recovery in FP64. */
@@ -86,7 +99,7 @@ typedef struct ox_config {
   int32_t slau_variant;  /* OX_SLAU_*                                           */
   int32_t rk;            /* RK: 3 (TVD, Shu-Osher) or 4 (low-storage)           */
   int32_t precision;     /* OX_PREC_*                                           */
-  int32_t bc_x, bc_y, bc_z; /* 0 = periodic; non-periodic is not supported    */
+  int32_t bc_x, bc_y, bc_z; /* OX_BC_*; non-periodic: Euler KEEP/SLAU/Hybrid   */
   int32_t device;        /* HIP device ordinal                                  */
   double gamma, R, Pr, Prt;
   double threshold;      /* hybrid switch on the Ducros sensor                  */
@@ -135,8 +148,9 @@ int ox_get_qj(ox_solver* s, double* QJ);
 
 /* Advances nsteps full RK steps (RK3: 3 stages, RK4: 4 stages).  Each stage
  * keeps the primitives of the ghost cells equal to those of their periodic
- * images, which is what the reference's per-stage set_bc achieves; the
- * state's own ghost layers are filled once at the end of the call.  Replaces
+ * images, or of the boundary fill of a non-periodic direction (OX_BC_*),
+ * which is what the reference's per-stage set_bc achieves; the state's own
+ * ghost layers are filled once at the end of the call.  Replaces
  * the inner `t1` loop of RungeKutta (3D_solver/src/calc_time_dev.f90.fypp:84-101
  * and :266-286). */
 int ox_step(ox_solver* s, int32_t nsteps);

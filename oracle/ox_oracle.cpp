@@ -25,6 +25,12 @@ bool Oracle::init(const ox_config& cf) {
   if (cf.visc != OX_VISC_EULER && vorder != 4 && vorder != 6) { err = "visc_order must be 4 or 6"; return false; }
   if (cf.visc != OX_VISC_EULER && vorder == 6 && order < 6) { err = "visc_order 6 needs order 6"; return false; }
   if (cf.scheme < OX_SCHEME_KEEP || cf.scheme > OX_SCHEME_HYBRID) { err = "unknown scheme"; return false; }
+  for (int b : {cf.bc_x, cf.bc_y, cf.bc_z})
+    if (b < OX_BC_PERIODIC || b > OX_BC_SLIP) { err = "unknown boundary kind"; return false; }
+  if ((cf.bc_x || cf.bc_y || cf.bc_z) && (cf.visc != OX_VISC_EULER || cf.scheme == OX_SCHEME_ROE)) {
+    err = "non-periodic boundaries: Euler KEEP, SLAU and Hybrid only";
+    return false;
+  }
   // ME4Base replaces the central-6 stencil only (calc_visc_high_internal.f90.fypp:17-23);
   // with VISC_ORDER 4 the central-4 stencil is used whatever VISC_STENCIL says
   me4 = cf.visc != OX_VISC_EULER && cf.visc_stencil == OX_STENCIL_ME4BASE && vorder == 6;
@@ -674,6 +680,10 @@ void Oracle::step4(std::vector<double>& Q) {  // :150-184
 // interior (j,k), y faces over interior (i,k), xy edges over interior k, then
 // whole z planes.
 void Oracle::set_bc(std::vector<double>& Q) const {
+  if (cfg.bc_x || cfg.bc_y || cfg.bc_z) {
+    set_bc_walls(Q);
+    return;
+  }
   const int h = off;
   const int ix = nx - 2 * h, iy = ny - 2 * h, iz = nz - 2 * h;
   for (int k = h + 1; k <= nz - h; ++k)
@@ -706,6 +716,42 @@ void Oracle::set_bc(std::vector<double>& Q) const {
           Q[iq(i, l, j, g)] = Q[iq(i, l, j, g + iz)];
           Q[iq(i, l, j, nz - h + g)] = Q[iq(i, l, j, h + g)];
         }
+}
+
+// A case's set_bc with non-periodic directions, written after
+// 3D_solver/STZ/set.f90:77-138: direction by direction in the order x, y, z,
+// each over the whole extent of the other two.  Per direction: periodic
+// (the image of the interior, as set_bc_cyclic), zero gradient (every ghost
+// layer copies the adjacent interior cell, :113-138) or slip wall (ghost
+// ovlp-m mirrors ovlp+1+m with the normal momentum negated, :77-112).
+void Oracle::set_bc_walls(std::vector<double>& Q) const {
+  const int h = off;
+  const int kinds[3] = {cfg.bc_x, cfg.bc_y, cfg.bc_z};
+  const int n[3] = {nx, ny, nz};
+  for (int d = 0; d < 3; ++d) {
+    const int N = n[d], m = N - 2 * h;
+    // the (1-based) source of ghost index g along d, and the sign of the normal momentum
+    auto src = [&](int g, double& sgn) {
+      sgn = 1.0;
+      if (kinds[d] == OX_BC_PERIODIC) return g <= h ? g + m : g - m;
+      if (kinds[d] == OX_BC_EXTRAPOLATE) return g <= h ? h + 1 : N - h;
+      sgn = -1.0;
+      return g <= h ? 2 * h + 1 - g : 2 * (N - h) + 1 - g;
+    };
+    for (int k = 1; k <= nz; ++k)
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+          const int p = d == 0 ? i : (d == 1 ? j : k);
+          if (p > h && p <= N - h) continue;
+          double sgn;
+          const int s = src(p, sgn);
+          const int si = d == 0 ? s : i, sj = d == 1 ? s : j, sk = d == 2 ? s : k;
+          for (int l = 1; l <= 5; ++l) {
+            const double v = Q[iq(si, l, sj, sk)];
+            Q[iq(i, l, j, k)] = (l == 2 + d && sgn < 0.0) ? -v : v;
+          }
+        }
+  }
 }
 
 // One step of RungeKutta (3D_solver/src/calc_time_dev.f90.fypp:84-101 for
@@ -871,6 +917,14 @@ int oxo_set_qj(void* hv, const double* QJ) {
 int oxo_get_qj(void* hv, double* QJ) {
   auto& o = static_cast<oxo_handle*>(hv)->o;
   std::memcpy(QJ, o.QJ.data(), o.nstate() * sizeof(double));
+  return 0;
+}
+
+// The case's ghost fill (set_bc) applied to the current state, as after a
+// stage (test hook).
+int oxo_set_bc(void* hv) {
+  auto& o = static_cast<oxo_handle*>(hv)->o;
+  o.set_bc(o.QJ);
   return 0;
 }
 

@@ -32,6 +32,8 @@
 //                                from the oracle's own right-hand sides;
 //                                time order
 //   periodic fill                ghost == periodic image, bitwise
+//   slip / zero-gradient fill    mirror images by hand; Sod's problem along z
+//                                between slip walls against the exact solution
 //   LES (SMS)                    off for a single mode, on for a multi-mode
 //   everything above             golden digests (tests/golden/)
 #pragma once
@@ -565,7 +567,8 @@ struct Oracle {
   void step2_3(double c1, double c2, double c3, double c4inv, const std::vector<double>& Qin,
                std::vector<double>& Qout);
   void step4(std::vector<double>& Q);
-  void set_bc(std::vector<double>& Q) const;  // set_bc_cyclic{2,4,6}
+  void set_bc(std::vector<double>& Q) const;  // set_bc_cyclic{2,4,6}, or the boundary fill below
+  void set_bc_walls(std::vector<double>& Q) const;  // STZ/set.f90:77-138 (slip walls, zero gradient)
   void rk_step();                             // one full RK step
   void rhs(std::vector<double>& R);           // calc_EFG + calc_R, interior only
   void diagnostics(ox_diag& d) const;

@@ -21,6 +21,7 @@ TVD = {"none": 0, "tvd": 1, "hybrid": 2}
 SLAU_VARIANT = {"SLAU": 0, "HRSLAU2": 1}
 STENCIL = {"Central": 0, "ME4Base": 1}
 PRECISION = {"fp64": 0, "fp32_bulk": 1}
+BC = {"periodic": 0, "extrapolate": 1, "slip": 2}  # OX_BC_* of one direction
 
 
 class OxConfig(ctypes.Structure):

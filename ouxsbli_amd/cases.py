@@ -58,6 +58,9 @@ class CaseSpec:
     Ly: float = 2.0 * math.pi
     Lz: float = 2.0 * math.pi
     device: int = 0
+    # boundary kind of x, y, z (both ends): "periodic", "extrapolate" (zero
+    # gradient) or "slip" (slip wall)
+    bc: tuple = ("periodic", "periodic", "periodic")
     # initial-condition constants (not part of ox_config)
     ic: str = "tgv"
     L0: float = 1.0
@@ -98,7 +101,7 @@ class CaseSpec:
         c.slau_variant = _abi.SLAU_VARIANT[self.slau_variant]
         c.rk = self.rk
         c.precision = _abi.PRECISION[self.precision]
-        c.bc_x = c.bc_y = c.bc_z = 0
+        c.bc_x, c.bc_y, c.bc_z = (_abi.BC[b] for b in self.bc)
         c.device = self.device
         c.gamma, c.R, c.Pr, c.Prt = self.gamma, self.R, self.Pr, self.Prt
         c.threshold = self.threshold
@@ -209,9 +212,46 @@ def tgv_state(spec: CaseSpec) -> np.ndarray:
     return periodic_fill(Q, spec.order)
 
 
+# ---------------------------------------------------------------------------
+# Shock tube along z between slip walls (the STZ case family of
+# 3D_solver/STZ: Euler SLAU (variant SLAU) ORDER 2, TVD, RK3; slip walls in x and y,
+# zero-gradient ends in z) on one domain instead of MPI z-slabs.
+# ---------------------------------------------------------------------------
+def stz_spec(nz: int = 256, nt: int = 4, order: int = 2, scheme: str = "SLAU", precision: str = "fp64",
+             rk: int = 3, cfl: float = 0.01) -> CaseSpec:
+    """STZ/mod_globals.f90: Lx = Ly = 0.1, Lz = 1, (rho, p) = (1, 1) | (0.125,
+    0.1), gamma 1.4, dt = CFL Lz / ((Nz - 1) sqrt(p_L / rho_L)) with Nz the
+    z extent; STZ/config.fypp: SLAU variant SLAU, TVD, ORDER 2, RK3; STZ/set.f90:77-138
+    walls.  `nz` and `nt` are the interior cells along z and across."""
+    Nz = nz + order
+    dt = cfl * 1.0 / ((Nz - 1) * math.sqrt(1.0 / 1.0))
+    return CaseSpec(nx=nt + order, ny=nt + order, nz=Nz, order=order, scheme=scheme, visc="Euler", tvd="tvd",
+                    slau_variant="SLAU", rk=rk, precision=precision, gamma=1.4, R=287.03, threshold=0.4, dt=dt,
+                    Lx=0.1, Ly=0.1, Lz=1.0, bc=("slip", "slip", "extrapolate"), ic="stz", RHO0=1.0, p0=1.0)
+
+
+STZ_LEFT = (1.0, 1.0)      # (rho_L, p_L), STZ/mod_globals.f90
+STZ_RIGHT = (0.125, 0.1)   # (rho_R, p_R)
+
+
+def stz_state(spec: CaseSpec) -> np.ndarray:
+    """STZ/set.f90:40-60 on one domain: the left state where the cell centre
+    is below Lz / 2, the right state above, at rest; ghost layers included."""
+    h = spec.order // 2
+    dz = spec.Lz / (spec.nz - spec.order)
+    zc = (np.arange(spec.nz) - h + 0.5) * dz
+    left = zc < 0.5 * spec.Lz
+    Q = np.zeros(spec.state_shape())
+    Q[:, :, 0, :] = np.where(left, STZ_LEFT[0], STZ_RIGHT[0])[:, None, None]
+    Q[:, :, 4, :] = (np.where(left, STZ_LEFT[1], STZ_RIGHT[1]) / (spec.gamma - 1.0))[:, None, None]
+    return Q
+
+
 def initial_state(spec: CaseSpec) -> np.ndarray:
     if spec.ic == "tgv":
         return tgv_state(spec)
+    if spec.ic == "stz":
+        return stz_state(spec)
     if spec.ic == "khi_tanh":
         return khi_state(spec)
     if spec.ic == "dhit":

@@ -439,6 +439,8 @@ template <class Init>
 int admit3d(ox_batch* b, const ox_config* cfg, Init init, int32_t nsteps, int64_t* id) {
   if (nsteps < 1) return ox_fail(OX_EINVAL, "nsteps must be >= 1");
   if (cfg->device != b->cfg.device) return ox_fail(OX_EINVAL, "case device differs from the batch device");
+  if (cfg->bc_x || cfg->bc_y || cfg->bc_z)
+    return ox_fail(OX_EUNSUPPORTED, "cases with non-periodic boundaries are not batched (use ox_step)");
   const size_t need = solver_footprint(*cfg);
   if (need > b->cfg.hbm_budget)
     return ox_fail(OX_EINVAL, "case footprint " + std::to_string(need) + " B exceeds the whole HBM budget " +

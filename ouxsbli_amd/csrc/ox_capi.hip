@@ -10,12 +10,30 @@
 #include <string>
 #include <vector>
 
+#include "ox_bc.hpp"
 #include "ox_core.hpp"
 #include "ox_diag.hpp"
 #include "ox_kernels.hpp"
 #include "ox_solver.hpp"
 
 namespace ox {
+
+// The state's ghost layers of a non-periodic direction d (the case's set_bc,
+// 3D_solver/STZ/set.f90:77-138, at the end of a step).
+__global__ __launch_bounds__(256) void k_bc_state(double* __restrict__ Q, Geom g, int d) {
+  int i, j, k;
+  if (!bc_cell(g, d, (long long)blockIdx.x * 256 + threadIdx.x, i, j, k)) return;
+  bool flip;
+  int si = i, sj = j, sk = k;
+  if (d == 0) si = bc_source(g.bc[0], i, g.h, g.ix, flip);
+  else if (d == 1) sj = bc_source(g.bc[1], j, g.h, g.iy, flip);
+  else sk = bc_source(g.bc[2], k, g.h, g.iz, flip);
+#pragma unroll
+  for (int l = 0; l < 5; ++l) {
+    const double v = Q[qidx(g, si, l, sj, sk)];
+    Q[qidx(g, i, l, j, k)] = (flip && l == 1 + d) ? -v : v;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Periodic ghost fill: every ghost cell takes its periodic image (the net
@@ -207,7 +225,13 @@ int validate(const ox_config& c) {
   if (c.abi_version != OX_ABI_VERSION) return ox_fail(OX_EINVAL, "abi_version mismatch");
   if (c.order != 2 && c.order != 4 && c.order != 6) return ox_fail(OX_EINVAL, "order must be 2, 4 or 6");
   if (c.rk != 3 && c.rk != 4) return ox_fail(OX_EINVAL, "rk must be 3 or 4");
-  if (c.bc_x || c.bc_y || c.bc_z) return ox_fail(OX_EUNSUPPORTED, "only periodic boundaries are built");
+  for (int b : {c.bc_x, c.bc_y, c.bc_z})
+    if (b != OX_BC_PERIODIC && b != OX_BC_EXTRAPOLATE && b != OX_BC_SLIP) return ox_fail(OX_EINVAL, "unknown boundary kind");
+  if ((c.bc_x || c.bc_y || c.bc_z) && (c.visc != OX_VISC_EULER || c.scheme == OX_SCHEME_ROE))
+    // the reference's non-periodic viscous closures are not built, and its
+    // non-periodic Roe kernels are first order on every face
+    // (3D_solver/src/calc_roe_kernel.f90.fypp:43-92)
+    return ox_fail(OX_EUNSUPPORTED, "non-periodic boundaries are built for Euler KEEP, SLAU and Hybrid");
   if (c.visc_stencil != OX_STENCIL_CENTRAL && c.visc_stencil != OX_STENCIL_ME4BASE)
     return ox_fail(OX_EINVAL, "unknown viscous stencil");
   if (c.scheme < OX_SCHEME_KEEP || c.scheme > OX_SCHEME_HYBRID) return ox_fail(OX_EINVAL, "unknown scheme");
@@ -241,6 +265,9 @@ ox::Geom make_geom(const ox_config& c) {
   g.a_xy = 0.25 * c.dt * (dx + dx) * (dy + dy);
   g.a_yz = 0.25 * c.dt * (dy + dy) * (dz + dz);
   g.a_zx = 0.25 * c.dt * (dz + dz) * (dx + dx);
+  g.bc[0] = c.bc_x;
+  g.bc[1] = c.bc_y;
+  g.bc[2] = c.bc_z;
   return g;
 }
 
@@ -280,6 +307,10 @@ void launch_ghost_fill(Core& C, double* Q) {
   const Geom& g = C.g;
   const long long total = 2LL * g.h * g.nx * g.ny + (long long)g.iz * 2 * g.h * g.nx + (long long)g.iz * g.iy * 2 * g.h;
   k_ghost_fill<<<(unsigned)((total + 255) / 256), 256, 0, C.stream>>>(Q, g);
+  // non-periodic directions: the case's ghost fill overwrites those layers
+  // (including the periodic images just written there)
+  for (int d = 0; d < 3; ++d)
+    if (g.bc[d]) k_bc_state<<<(unsigned)((bc_cells(g, d) + 255) / 256), 256, 0, C.stream>>>(Q, g, d);
 }
 }  // namespace ox
 

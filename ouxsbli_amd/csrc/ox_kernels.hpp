@@ -844,9 +844,12 @@ __device__ __forceinline__ void sweep_line_tile(const SweepArgs<R, S>& a, int bx
   // the sensor is stored for interior cells only; a ghost cell's sensor is
   // bitwise that of its periodic image (its inputs are exact copies), except
   // with one ghost layer (ORDER 2), where the reference copies the adjacent
-  // interior value into the ghost cell (calc_hybrid.f90:54-71)
+  // interior value into the ghost cell (calc_hybrid.f90:54-71), and along a
+  // non-periodic direction, where the inner ghost layer holds its own sensor
+  const bool periodic = g.bc[DIR] == OX_BC_PERIODIC;
   auto wrap = [&](int sp) {
     if constexpr (ORDER == 2) return sp < g.h ? g.h : (sp >= g.h + nd ? g.h + nd - 1 : sp);
+    if (!periodic) return sp;
     return sp < g.h ? sp + nd : (sp >= g.h + nd ? sp - nd : sp);
   };
   using RC = RawCell<S, VISC, VO>;

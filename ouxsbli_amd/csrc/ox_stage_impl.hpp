@@ -7,6 +7,7 @@
 // uniform runtime switches.
 #pragma once
 
+#include "ox_bc.hpp"
 #include "ox_core.hpp"
 #include "ox_kernels.hpp"
 
@@ -45,6 +46,7 @@ void prepare_stage(Core& C, const double* Qsrc) {
     k_prim<S, (VISC != 0)><<<dim3(cdiv(g.nx, 64), cdiv(g.ny, 4), g.nz), blk, 0, C.stream>>>(
         Qsrc, P, g, make_physc<double>(C.hc));
     C.prim_for = Qsrc;
+    if (any_bc(g)) launch_bc_sensors<R, S, ORDER, SCHEME>(g, P, C.stream);
     prof_mark(C, "prim");
   }
   if constexpr (VISC == 2) {
@@ -102,6 +104,13 @@ void run_stage(Core& C, const double* Qsrc, const StageArgs& sa, double cx, doub
   }
 #undef OX_Z
   prof_mark(C, "sweep_z_rk");
+  if (sa.ep != EP_RHS && any_bc(g)) {
+    // the case's set_bc on the new state, as ghost primitives (and sensors)
+    const Prim<S> Pn = prim_set<S>(C, 1 - C.cur);
+    launch_bc_prims<R, S, ORDER, SCHEME, VISC>(g, Pn, C.stream);
+    launch_bc_sensors<R, S, ORDER, SCHEME>(g, Pn, C.stream);
+    prof_mark(C, "bc");
+  }
   finish_stage(C, sa);
 }
 

@@ -75,6 +75,7 @@ struct Geom {
   double jac;            // Jacobian(i,j) = 8/((dx+dx)(dy+dy)*2dz)
   double inv_dx, inv_dy, inv_dz;
   double a_xy, a_yz, a_zx;  // dt * face areas (pre_calc dtdxdy, dtdydz, dtdzdx)
+  int bc[3];             // OX_BC_* of x, y, z
 };
 
 // Stage epilogue of the last sweep (3D_solver/src/calc_steps.f90):

@@ -34,6 +34,8 @@ def lib():
             getattr(L, f).restype = ctypes.c_int
         L.oxo_step.argtypes = [P, ctypes.c_int]
         L.oxo_step.restype = ctypes.c_int
+        L.oxo_set_bc.argtypes = [P]
+        L.oxo_set_bc.restype = ctypes.c_int
         L.oxo_get_primitives.argtypes = [P, D, D, D, D]
         L.oxo_diagnostics.argtypes = [P, ctypes.POINTER(_abi.OxDiag)]
         L.oxo_get_les.argtypes = [P, D, D]
@@ -105,6 +107,10 @@ class Oracle:
 
     def step(self, n=1):
         lib().oxo_step(self.h, int(n))
+
+    def set_bc(self):
+        """The case's ghost fill on the current state (as after a stage)."""
+        lib().oxo_set_bc(self.h)
 
     def rhs(self):
         R = np.empty(self.spec.state_shape())

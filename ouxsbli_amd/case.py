@@ -27,7 +27,7 @@ from . import io_vtr
 from .cases import CaseSpec, cell_centres
 from .cases2d import CaseSpec2D
 from .fortran_params import read_fypp_macros, read_parameters
-from .ic import bc_to_params, device_ic, ic_to_params, initial_state
+from .ic import CASE_BOUNDARIES, bc_to_params, device_ic, ic_to_params, initial_state
 from .patcher import _is_macro_line, patch
 
 _ALIAS = {
@@ -119,11 +119,19 @@ class Case:
         m, g = self.macros, self.globals
         if int(g.get("dimension", 3)) == 2:
             return self._make_spec2d()
-        for flag in ("BC_X", "BC_Y", "BC_Z"):
-            if m.get(flag, False):
-                raise RuntimeError(f"{flag}=True: only periodic boundaries are built into the gfx950 library")
-        if m.get("COMMZ", False):
-            raise RuntimeError("COMMZ=True: one GPU holds the whole grid; z-slab decomposition is not built")
+        # BC_*=True selects the reference's order-fallback flux kernels, whose
+        # faces next to interior cells are the periodic kernels' own; what
+        # differs is the case's ghost fill (its set_bc), built for the cases
+        # of CASE_BOUNDARIES.  Their z slabs (COMMZ) are one domain here.
+        bc = CASE_BOUNDARIES.get(self.source.name)
+        if bc is None:
+            for flag in ("BC_X", "BC_Y", "BC_Z"):
+                if m.get(flag, False):
+                    raise RuntimeError(f"{flag}=True: the boundary fill of case {self.source.name!r} is not built "
+                                       "(non-periodic fills: " + ", ".join(CASE_BOUNDARIES) + ")")
+            if m.get("COMMZ", False):
+                raise RuntimeError("COMMZ=True: one GPU holds the whole grid; z-slab decomposition is not built")
+            bc = ("periodic",) * 3
         if m.get("RESCALE", False):
             raise RuntimeError("RESCALE=True (SBLI inflow recycling) is not part of this library")
         order = int(m.get("ORDER", 6))
@@ -134,7 +142,7 @@ class Case:
             slau_variant=str(m.get("SLAU_VARIANT", "HRSLAU2")), rk=int(m.get("RK", 3)),
             precision=str(m.get("PRECISION", "fp64")), gamma=float(g.get("gamma", 1.4)), R=float(g.get("r", 287.03)),
             Pr=float(g.get("pr", 0.71)), Prt=float(g.get("prt", 0.9)), threshold=float(g.get("threshold", 0.4)),
-            dt=float(g["dt"]), Lx=float(g["lx"]), Ly=float(g["ly"]), Lz=float(g["lz"]))
+            dt=float(g["dt"]), Lx=float(g["lx"]), Ly=float(g["ly"]), Lz=float(g["lz"]), bc=bc)
         if spec.visc == "Euler":
             spec.visc_order = 0
         return spec

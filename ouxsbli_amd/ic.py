@@ -178,7 +178,25 @@ def etgv_state(spec, g):
     return periodic_fill(Q, spec.order)
 
 
-CASE_STATES = {"NSTGV": nstgv_state, "ETGV": etgv_state, "KHI": khi_state}
+def stz_state(spec, g):
+    """3D_solver/STZ/set.f90:40-60 on one domain: (rho_L, p_L) below the
+    diaphragm at Lz / 2 (the first half of the reference's z slabs), (rho_R,
+    p_R) above, at rest, ghost layers included."""
+    Z, Y, X = _grid(spec)
+    left = Z < 0.5 * spec.Lz
+    rho = np.where(left, g.get("rho_l", 1.0), g.get("rho_r", 0.125))
+    p = np.where(left, g.get("p_l", 1.0), g.get("p_r", 0.1))
+    Q = np.zeros(spec.state_shape())  # set_init writes the ghost cells too: no periodic fill
+    Q[:, :, 0, :] = rho
+    Q[:, :, 4, :] = p / (spec.gamma - 1.0)
+    return Q
+
+
+CASE_STATES = {"NSTGV": nstgv_state, "ETGV": etgv_state, "KHI": khi_state, "STZ": stz_state}
+
+# Boundary kinds (x, y, z) of the cases whose set_bc is not the periodic fill:
+# STZ/set.f90:77-138 (slip walls in x and y, zero-gradient z ends).
+CASE_BOUNDARIES = {"STZ": ("slip", "slip", "extrapolate")}
 
 
 def device_ic(case_name: str, spec, g: dict):

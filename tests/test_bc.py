@@ -236,3 +236,21 @@ def test_nonperiodic_refused_where_not_built(kw):
         setattr(spec, k, v)
     with pytest.raises(RuntimeError):
         Oracle(spec)
+
+
+def test_stz_case_builds_with_its_walls(tmp_path):
+    """Case('3D_solver/STZ') reads the case's switches and parameters
+    (SLAU variant SLAU, TVD, ORDER 2, RK3) and its boundary kinds; other 3-D
+    cases keep refusing BC_*=True (their ghost fill is the periodic one)."""
+    from ouxsbli_amd.case import Case
+    from ouxsbli_amd.ic import initial_state
+    c = Case("3D_solver/STZ", str(tmp_path / "stz"))
+    c.build()
+    s = c.spec
+    assert s.bc == ("slip", "slip", "extrapolate")
+    assert (s.order, s.scheme, s.slau_variant, s.tvd, s.rk, s.visc) == (2, "SLAU", "SLAU", "tvd", 3, "Euler")
+    ref = cases.stz_spec(256, 4)
+    assert (s.nx, s.ny, s.nz, s.Lx, s.Ly, s.Lz, s.dt) == (ref.nx, ref.ny, ref.nz, ref.Lx, ref.Ly, ref.Lz, ref.dt)
+    np.testing.assert_array_equal(initial_state("STZ", s, c.globals), cases.stz_state(ref))
+    with pytest.raises(RuntimeError):
+        Case("3D_solver/KHI", str(tmp_path / "k"), bc_y=True).build()

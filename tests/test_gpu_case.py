@@ -84,3 +84,24 @@ def test_oblique_shock_case_like_the_reference_test(tmp_path):
     recal = tmp_path / "os" / "recal"
     assert (recal / "Q00001.dat").stat().st_size == nx * 4 * ny * 8
     assert (recal / "x.dat").stat().st_size == nx * 8
+
+
+def test_stz_case_between_walls(tmp_path):
+    """The STZ case (Sod's problem along z between slip walls, zero-gradient
+    ends) through the Case API: its 50 outputs, and the final state against
+    the exact solution, as on the solver (tests/test_gpu_bc.py)."""
+    from ouxsbli_amd import cases
+    from test_bc import STZ_L1, stz_l1
+    c = Case("3D_solver/STZ", str(tmp_path / "stz"))
+    c.build()
+    assert c.spec.bc == ("slip", "slip", "extrapolate")
+    c.run()
+    d = tmp_path / "stz" / "data"
+    n_out = int(c.globals["np"])
+    assert (d / f"Q{n_out:05d}.vtr").exists()
+    assert len((d / "kinetic_energy.d").read_text().splitlines()) == n_out + 1
+    Q = io_vtr.read_restart(tmp_path / "stz" / "recal" / "Q00001.dat", c.spec.state_shape())
+    steps = n_out * int(c.globals["nt"])
+    l1_rho, l1_p = stz_l1(Q, c.spec, steps * c.spec.dt)
+    assert l1_rho < STZ_L1 and l1_p < STZ_L1, (l1_rho, l1_p)
+    assert cases.stz_spec(256, 4).dt == c.spec.dt

@@ -99,7 +99,7 @@ typedef struct ox_config {
   int32_t slau_variant;  /* OX_SLAU_*                                           */
   int32_t rk;            /* RK: 3 (TVD, Shu-Osher) or 4 (low-storage)           */
   int32_t precision;     /* OX_PREC_*                                           */
-  int32_t bc_x, bc_y, bc_z; /* OX_BC_*; non-periodic: Euler KEEP/SLAU/Hybrid   */
+  int32_t bc_x, bc_y, bc_z; /* OX_BC_*; non-periodic: Euler/NS, not Roe or LES */
   int32_t device;        /* HIP device ordinal                                  */
   double gamma, R, Pr, Prt;
   double threshold;      /* hybrid switch on the Ducros sensor                  */

@@ -27,8 +27,8 @@ bool Oracle::init(const ox_config& cf) {
   if (cf.scheme < OX_SCHEME_KEEP || cf.scheme > OX_SCHEME_HYBRID) { err = "unknown scheme"; return false; }
   for (int b : {cf.bc_x, cf.bc_y, cf.bc_z})
     if (b < OX_BC_PERIODIC || b > OX_BC_SLIP) { err = "unknown boundary kind"; return false; }
-  if ((cf.bc_x || cf.bc_y || cf.bc_z) && (cf.visc != OX_VISC_EULER || cf.scheme == OX_SCHEME_ROE)) {
-    err = "non-periodic boundaries: Euler KEEP, SLAU and Hybrid only";
+  if ((cf.bc_x || cf.bc_y || cf.bc_z) && (cf.visc == OX_VISC_LES || cf.scheme == OX_SCHEME_ROE)) {
+    err = "non-periodic boundaries: Euler and NS with KEEP, SLAU and Hybrid only";
     return false;
   }
   // ME4Base replaces the central-6 stencil only (calc_visc_high_internal.f90.fypp:17-23);

@@ -227,11 +227,12 @@ int validate(const ox_config& c) {
   if (c.rk != 3 && c.rk != 4) return ox_fail(OX_EINVAL, "rk must be 3 or 4");
   for (int b : {c.bc_x, c.bc_y, c.bc_z})
     if (b != OX_BC_PERIODIC && b != OX_BC_EXTRAPOLATE && b != OX_BC_SLIP) return ox_fail(OX_EINVAL, "unknown boundary kind");
-  if ((c.bc_x || c.bc_y || c.bc_z) && (c.visc != OX_VISC_EULER || c.scheme == OX_SCHEME_ROE))
-    // the reference's non-periodic viscous closures are not built, and its
+  if ((c.bc_x || c.bc_y || c.bc_z) && (c.visc == OX_VISC_LES || c.scheme == OX_SCHEME_ROE))
+    // an LES case's ghost fill of nu_t and qc2 next to walls is its own
+    // set_bc_mut (3D_solver/TBL/set.f90:142-185), not built; the reference's
     // non-periodic Roe kernels are first order on every face
     // (3D_solver/src/calc_roe_kernel.f90.fypp:43-92)
-    return ox_fail(OX_EUNSUPPORTED, "non-periodic boundaries are built for Euler KEEP, SLAU and Hybrid");
+    return ox_fail(OX_EUNSUPPORTED, "non-periodic boundaries are built for Euler and NS with KEEP, SLAU and Hybrid");
   if (c.visc_stencil != OX_STENCIL_CENTRAL && c.visc_stencil != OX_STENCIL_ME4BASE)
     return ox_fail(OX_EINVAL, "unknown viscous stencil");
   if (c.scheme < OX_SCHEME_KEEP || c.scheme > OX_SCHEME_HYBRID) return ox_fail(OX_EINVAL, "unknown scheme");

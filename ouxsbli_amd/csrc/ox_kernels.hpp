@@ -846,11 +846,12 @@ __device__ __forceinline__ void sweep_line_tile(const SweepArgs<R, S>& a, int bx
   // with one ghost layer (ORDER 2), where the reference copies the adjacent
   // interior value into the ghost cell (calc_hybrid.f90:54-71), and along a
   // non-periodic direction, where the inner ghost layer holds its own sensor
-  const bool periodic = g.bc[DIR] == OX_BC_PERIODIC;
+  // (the shift is a select, not a branch: a branch in the loop would make the
+  // compiler wait for every load in flight at its merge point)
+  const int period = g.bc[DIR] == OX_BC_PERIODIC ? nd : 0;
   auto wrap = [&](int sp) {
     if constexpr (ORDER == 2) return sp < g.h ? g.h : (sp >= g.h + nd ? g.h + nd - 1 : sp);
-    if (!periodic) return sp;
-    return sp < g.h ? sp + nd : (sp >= g.h + nd ? sp - nd : sp);
+    return sp < g.h ? sp + period : (sp >= g.h + nd ? sp - period : sp);
   };
   using RC = RawCell<S, VISC, VO>;
   CD win[St::SW];

@@ -25,14 +25,23 @@ CASES = {
     "slau6-muscl-y-z-slip": ((60, 72, 40), 6, "SLAU", "tvd", "HRSLAU2", 3, ("extrapolate", "slip", "slip"), 3),
     "keep4-rk4-walls": ((20, 24, 28), 4, "KEEP", "none", "HRSLAU2", 4, ("slip", "slip", "slip"), 4),
     "slau4-y-open": ((36, 30, 20), 4, "SLAU", "tvd", "SLAU", 3, ("periodic", "extrapolate", "periodic"), 4),
+    # Navier-Stokes: config 2's scheme between slip walls in y and z, and
+    # ME4Base / central-4 variants
+    "slau6-ns6-y-z-slip": ((130, 40, 72), 6, "SLAU", "hybrid", "HRSLAU2", 3, ("periodic", "slip", "slip"), 2,
+                           "NS", 6, "Central"),
+    "hybrid6-me4-walls": ((60, 30, 24), 6, "Hybrid", "hybrid", "HRSLAU2", 3, ("slip", "extrapolate", "slip"), 2,
+                          "NS", 6, "ME4Base"),
+    "keep4-ns4-x-open": ((40, 20, 24), 4, "KEEP", "none", "HRSLAU2", 4, ("extrapolate", "periodic", "slip"), 3,
+                         "NS", 4, "Central"),
 }
 
 
 def _case(name, precision="fp64"):
-    (nx, ny, nz), order, scheme, tvd, variant, rk, bc, steps = CASES[name]
-    spec = CaseSpec(nx=nx + order, ny=ny + order, nz=nz + order, order=order, scheme=scheme, visc="Euler", tvd=tvd,
-                    slau_variant=variant, rk=rk, precision=precision, threshold=0.4, dt=1e-3, Lx=1.0, Ly=0.9,
-                    Lz=1.1, bc=bc)
+    (nx, ny, nz), order, scheme, tvd, variant, rk, bc, steps, *visc = CASES[name]
+    visc, vorder, stencil = visc if visc else ("Euler", 0, "Central")
+    spec = CaseSpec(nx=nx + order, ny=ny + order, nz=nz + order, order=order, scheme=scheme, visc=visc,
+                    visc_order=vorder, visc_stencil=stencil, tvd=tvd, slau_variant=variant, rk=rk,
+                    precision=precision, threshold=0.4, dt=1e-3, Lx=1.0, Ly=0.9, Lz=1.1, bc=bc)
     h = order // 2
     Q = np.zeros(spec.state_shape())
     Q[h:-h, h:-h, :, h:-h] = _field((nz, ny, nx), seed=len(name))
@@ -105,7 +114,7 @@ def test_stz_fp64_equals_oracle():
         np.testing.assert_array_equal(s.get_qj(), o.get_qj())
 
 
-@pytest.mark.parametrize("kw", [dict(scheme="Roe"), dict(visc="NS", visc_order=4), dict(visc="LES", visc_order=4)])
+@pytest.mark.parametrize("kw", [dict(scheme="Roe"), dict(visc="LES", visc_order=4)])
 def test_walls_refused_where_not_built(kw):
     spec = cases.stz_spec(nz=32, nt=16, order=4)
     for k, v in kw.items():

@@ -99,6 +99,14 @@ struct ox_batch {
   std::vector<int> launch_stage;
   std::vector<int> launch_grp;
   std::vector<size_t> args_off;                // per launch: byte offset of its x,y,z descriptor arrays
+  // boundary fills of the cases with non-periodic directions, per descriptor
+  // set, in launch order: after launch `li`, the fill of one case
+  struct BcFill {
+    size_t li;
+    ox::Geom g;
+    void* pn[10];  // the primitive set that launch's epilogue writes
+  };
+  std::vector<BcFill> bc_fills[2];
   size_t args_bytes = 0;     // one descriptor set (all launches of one step)
   char* d_args = nullptr;    // two sets: set k at d_args + k * args_bytes
   size_t d_args_cap = 0;
@@ -335,6 +343,7 @@ int build_plan(ox_batch* b) {
       }
   };
   for (int set = 0; set < 2; ++set) {
+    b->bc_fills[set].clear();
     for (size_t li = 0; li < b->launches.size(); ++li) {
       BGroup& G = b->groups[b->launch_grp[li]];
       if (G.dim == 2) continue;
@@ -357,6 +366,13 @@ int build_plan(ox_batch* b) {
         G.ops.make_args(s->C, sa, coef * g.a_yz, coef * g.a_zx, coef * g.a_xy, tmp);
         for (int d = 0; d < 3; ++d) std::memcpy(base + ((size_t)d * n + m) * ab, tmp + (size_t)d * ab, ab);
         b->launches[li].ep = sa.ep;
+        if ((g.bc[0] || g.bc[1] || g.bc[2]) && sa.ep != ox::EP_RHS) {
+          ox_batch::BcFill f;
+          f.li = li;
+          f.g = g;
+          for (int m2 = 0; m2 < 10; ++m2) f.pn[m2] = s->C.prim[1 - s->C.cur][m2];
+          b->bc_fills[set].push_back(f);
+        }
         ox::finish_stage(s->C, sa);
       }
     }
@@ -405,6 +421,8 @@ int tune_case(ox_batch* b, ox_solver* s) {
 
 // The launches of one step reading descriptor set `set`.
 void enqueue_step(ox_batch* b, int set) {
+  size_t fi = 0;
+  const std::vector<ox_batch::BcFill>& fills = b->bc_fills[set];
   for (size_t li = 0; li < b->launches.size(); ++li) {
     const BGroup& G = b->groups[b->launch_grp[li]];
     ox::BatchGroupLaunch L = b->launches[li];
@@ -413,6 +431,8 @@ void enqueue_step(ox_batch* b, int set) {
       const size_t n = G.members.size();
       for (int d = 0; d < 3; ++d) L.args[d] = base + (size_t)d * n * G.ops.args_bytes;
       G.ops.launch(L);
+      // the boundary fills of this stage's non-periodic cases
+      for (; fi < fills.size() && fills[fi].li == li; ++fi) G.ops.bc_after(fills[fi].g, fills[fi].pn, L.stream);
     } else {
       G.launch2[b->launch_stage[li]].launch(base, L.ncase, L.ntiles[0], L.grid[0], L.stream);
     }
@@ -439,8 +459,6 @@ template <class Init>
 int admit3d(ox_batch* b, const ox_config* cfg, Init init, int32_t nsteps, int64_t* id) {
   if (nsteps < 1) return ox_fail(OX_EINVAL, "nsteps must be >= 1");
   if (cfg->device != b->cfg.device) return ox_fail(OX_EINVAL, "case device differs from the batch device");
-  if (cfg->bc_x || cfg->bc_y || cfg->bc_z)
-    return ox_fail(OX_EUNSUPPORTED, "cases with non-periodic boundaries are not batched (use ox_step)");
   const size_t need = solver_footprint(*cfg);
   if (need > b->cfg.hbm_budget)
     return ox_fail(OX_EINVAL, "case footprint " + std::to_string(need) + " B exceeds the whole HBM budget " +

@@ -92,6 +92,10 @@ struct BatchOps {
   // the three sweep descriptors (x, y, z) of one case for one stage
   void (*make_args)(Core& C, const StageArgs& sa, double cx, double cy, double cz, void* out3);
   void (*launch)(const BatchGroupLaunch& L);
+  // after a stage of one case with non-periodic directions: the ghost
+  // primitives of the set its epilogue wrote (prims_new, 10 arrays), then the
+  // Ducros sensors of the inner ghost layers
+  void (*bc_after)(const Geom& g, void* const* prims_new, hipStream_t st);
 };
 
 bool find_batch_fp64(int order, int scheme, int visc, int vorder, BatchOps* ops);

@@ -149,6 +149,15 @@ void make_sweep_args_erased(Core& C, const StageArgs& sa, double cx, double cy, 
   make_sweep_args<R, S>(C, sa, cx, cy, cz, (SweepArgs<R, S>*)out3);
 }
 
+template <class R, class S, int ORDER, int SCHEME, int VISC>
+void bc_after_erased(const Geom& g, void* const* pn, hipStream_t st) {
+  Prim<S> P;
+  S** a[10] = {&P.rho, &P.u, &P.v, &P.w, &P.p, &P.T, &P.mu, &P.s, &P.mut, &P.qc2};
+  for (int m = 0; m < 10; ++m) *a[m] = (S*)pn[m];
+  launch_bc_prims<R, S, ORDER, SCHEME, VISC>(g, P, st);
+  launch_bc_sensors<R, S, ORDER, SCHEME>(g, P, st);
+}
+
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO> BatchOps batch_ops() {
   BatchOps o;
   o.args_bytes = sizeof(SweepArgs<R, S>);
@@ -156,6 +165,7 @@ template <class R, class S, int ORDER, int SCHEME, int VISC, int VO> BatchOps ba
   o.prepare = &prepare_stage<R, S, ORDER, SCHEME, VISC, VO>;
   o.make_args = &make_sweep_args_erased<R, S>;
   o.launch = &run_stage_batch<R, S, ORDER, SCHEME, VISC, VO>;
+  o.bc_after = &bc_after_erased<R, S, ORDER, SCHEME, VISC>;
   return o;
 }
 

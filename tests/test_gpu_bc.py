@@ -123,8 +123,34 @@ def test_walls_refused_where_not_built(kw):
         Solver(spec)
 
 
-def test_walls_not_batched():
-    spec = cases.stz_spec(32, 8)
-    with Batch(hbm_budget=1 << 30, max_cases=2) as b:
-        with pytest.raises(RuntimeError):
-            b.admit(spec, cases.stz_state(spec), 2)
+@pytest.mark.parametrize("precision", ["fp64", "fp32_bulk"])
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_walls_batched_equal_alone(precision, use_graph):
+    """Cases with non-periodic directions in the batch engine (their
+    boundary fills follow their group's stage, captured in the step graphs)
+    beside periodic cases of the same specialisation: each ends bit-identical
+    to its ox_step run."""
+    mix = []
+    for name in ("slau6-ns6-y-z-slip", "keep4-rk4-walls", "hybrid6-me4-walls"):
+        spec, Q, _ = _case(name, precision)
+        mix.append((spec, Q, 3))
+    stz = cases.stz_spec(64, 4, precision=precision)
+    mix.append((stz, cases.stz_state(stz), 5))
+    per = cases.tgv_spec(20, 6, Ma=1.25, cfl=0.3, cfl_mode="acoustic", scheme="SLAU", visc="NS", visc_order=6,
+                         rk=3, tvd="hybrid", precision=precision)
+    mix.append((per, cases.initial_state(per), 4))
+    ref = []
+    for spec, Q, n in mix:
+        with Solver(spec) as s:
+            s.set_state(Q)
+            s.step(n)
+            ref.append(s.get_state())
+    out = {}
+    with Batch(hbm_budget=8 << 30, max_cases=8, use_graph=use_graph, autotune=False) as b:
+        ids = {b.admit(spec, Q, n): k for k, (spec, Q, n) in enumerate(mix)}
+        while len(out) < len(mix):
+            b.run(10)
+            for cid in b.done():
+                out[ids[cid]] = b.retire(cid)
+    for k in range(len(mix)):
+        np.testing.assert_array_equal(out[k], ref[k], err_msg=f"case {k}")

@@ -105,3 +105,20 @@ def test_stz_case_between_walls(tmp_path):
     l1_rho, l1_p = stz_l1(Q, c.spec, steps * c.spec.dt)
     assert l1_rho < STZ_L1 and l1_p < STZ_L1, (l1_rho, l1_p)
     assert cases.stz_spec(256, 4).dt == c.spec.dt
+
+
+def test_etgv_case_like_the_reference_test(tmp_path):
+    """The reference's ETGV test (ouxsbli/tests/test_etgv.py:15-60) through
+    the Case API with its own parameters (KEEP ORDER 2, Euler, RK4, 66^3,
+    20000 steps of dt 0.01, one output) and its thresholds on the .d files it
+    reads: |(s0-s)/s0| < 3e-3 and 0.75 < ke/ke0 < 1.01."""
+    case = Case("3D_solver/ETGV", str(tmp_path / "etgv"), visc="euler", scheme="keep", accuracy=2, rk="rk4",
+                nx=66, ny=66, nz=66, nt=20000, np=1, dt=0.01e0)
+    case.build()
+    case.run(nranks=2)
+    d = tmp_path / "etgv" / "data"
+    ent = np.loadtxt(d / "entropy.d")
+    ke = np.loadtxt(d / "kinetic_energy.d")
+    assert ent.shape == (2, 2) and ke.shape == (2, 3)
+    assert abs(ent[-1, -1]) < 3.0e-3, ent
+    assert 0.75 < ke[-1, -1] < 1.0 + 0.01, ke

@@ -131,7 +131,7 @@ def _oracle_rate(spec, nsteps, repeats):
 def cpu_baseline(n_small=64, n_large=128):
     """The FP64 OpenMP oracle (the parity build) on the host cores, on bounded
     samples: the headline scheme at 64^3 (10 steps, median of 5) and 128^3
-    (2 steps, median of 3), and configuration 1 in full (64^3 KEEP4 NS RK4,
+    (2 steps, median of 3), and the 64^3 KEEP4 NS TGV in full (RK4,
     100 steps).  OpenMP threads are bound to cores (OMP_PROC_BIND=close,
     OMP_PLACES=cores) before the oracle library loads; OMP_NUM_THREADS keeps
     the value the environment sets, else one thread per physical core."""
@@ -142,11 +142,11 @@ def cpu_baseline(n_small=64, n_large=128):
     sys.path.insert(0, str(ROOT / "tests"))
     from ouxsbli_amd import cases
     t0 = time.perf_counter()
-    v64, r64 = _oracle_rate(cases.config2_spec(n_small, "fp64"), 10, 5)
-    v128, r128 = _oracle_rate(cases.config2_spec(n_large, "fp64"), 2, 3)
-    # configuration 1 in full
+    v64, r64 = _oracle_rate(cases.supersonic_tgv_spec(n_small, "fp64"), 10, 5)
+    v128, r128 = _oracle_rate(cases.supersonic_tgv_spec(n_large, "fp64"), 2, 3)
+    # the 64^3 KEEP4 NS TGV in full
     import oracle_lib
-    c1 = cases.config1_spec(64)
+    c1 = cases.tgv_keep4_spec(64)
     o = oracle_lib.Oracle(c1)
     o.set_state(cases.initial_state(c1))
     tc = time.perf_counter()
@@ -161,7 +161,7 @@ def cpu_baseline(n_small=64, n_large=128):
             "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
             "headline_scheme_64": {"value": round(v64, 2), "steps": 10, "repeats": [round(r, 2) for r in r64]},
             "headline_scheme_128": {"value": round(v128, 2), "steps": 2, "repeats": [round(r, 2) for r in r128]},
-            "config1_full": {"value": round(c1.interior_cells * c1.stages * 100 / t_c1 / 1e6, 2),
+            "tgv64_keep4_full": {"value": round(c1.interior_cells * c1.stages * 100 / t_c1 / 1e6, 2),
                              "seconds": round(t_c1, 2),
                              "workload": "NS TGV 64^3 KEEP4 central-4 viscous RK4 Ma 0.1, 100 steps"}}
 
@@ -240,7 +240,7 @@ def batched_os_leg(ncase, steps, unbatched_steps, cpu_seconds):
            "cases": ncase, "steps": steps}
     res = {}
     for prec in ("fp32_bulk", "fp64"):
-        specs = c2.config4_specs(ncase, precision=prec)
+        specs = c2.oblique_shock_specs(ncase, precision=prec)
         states = [c2.os_state(s) for s in specs]
         cs = sum(s.interior_cells for s in specs) * 3 * steps
         with Batch(hbm_budget=32 << 30, max_cases=ncase, use_graph=True, autotune=False) as b:
@@ -282,7 +282,7 @@ def batched_os_leg(ncase, steps, unbatched_steps, cpu_seconds):
     # the same cases one at a time through the plain single-case kernels (one
     # launch per stage per case), in both forms
     for prec in ("fp32_bulk", "fp64"):
-        specs = c2.config4_specs(ncase, precision=prec)
+        specs = c2.oblique_shock_specs(ncase, precision=prec)
         t_dev = 0.0
         cs1 = 0
         for s in specs:
@@ -367,7 +367,7 @@ def serving_leg(n_requests, fp64_value):
     out["fp32_vs_fp64_same_requests_wall"] = round(warm["fp32_bulk"]["Mcell_updates_per_s_stage"] /
                                                    warm["fp64"]["Mcell_updates_per_s_stage"], 3)
     busy = cold["device_busy_Mcell_updates_per_s_stage"]
-    out["device_busy_vs_fp64_plain_config2"] = round(busy / fp64_value, 3) if busy else None
+    out["device_busy_vs_fp64_plain_supersonic_tgv"] = round(busy / fp64_value, 3) if busy else None
     return out
 
 
@@ -425,7 +425,7 @@ def run_form(spec, steps, warmup, profile):
 
 
 def history_leg(n, nsteps, every):
-    """Configuration 2's kinetic-energy and enstrophy histories: the FP64
+    """The 256^3 supersonic TGV's kinetic-energy and enstrophy histories: the FP64
     plain and the FP32-bulk forms from the same initial field, diagnostics
     every `every` steps over `nsteps` steps (the FP64 GPU form stands in for
     the oracle at this size; it equals the oracle bit for bit on the same
@@ -434,11 +434,11 @@ def history_leg(n, nsteps, every):
     excluded."""
     from ouxsbli_amd import cases
     from ouxsbli_amd.solver import Solver
-    out = {"workload": f"config 2 ({n}^3 supersonic NS TGV, SLAU6 HR-SLAU2 hybrid NS6 RK3), {nsteps} steps",
+    out = {"workload": f"{n}^3 supersonic NS TGV (SLAU6 HR-SLAU2 hybrid NS6 RK3), {nsteps} steps",
            "every": every}
     hist = {}
     for prec in ("fp64", "fp32_bulk"):
-        spec = cases.config2_spec(n, prec)
+        spec = cases.supersonic_tgv_spec(n, prec)
         rows = []
         dev_ms = 0.0
         t0 = time.perf_counter()
@@ -505,8 +505,8 @@ def main():
         return
 
     from ouxsbli_amd import cases
-    spec = cases.config2_spec(args.n, "fp64")
-    spec32 = cases.config2_spec(args.n, "fp32_bulk")
+    spec = cases.supersonic_tgv_spec(args.n, "fp64")
+    spec32 = cases.supersonic_tgv_spec(args.n, "fp32_bulk")
     ops = _opcounts()["slau6_hrslau2_hybrid_ns6_rk3"]
 
     r64 = run_form(spec, args.steps, args.warmup, True)
@@ -588,7 +588,7 @@ def main():
         "state_ok": bool(r64["ok"] and r32["ok"]),
     }
     if args.me4:
-        spec_me4 = cases.config2_spec(args.n, "fp64")
+        spec_me4 = cases.supersonic_tgv_spec(args.n, "fp64")
         spec_me4.visc_stencil = "ME4Base"
         r_me4 = run_form(spec_me4, args.steps, args.warmup, False)
         v_me4 = spec_me4.interior_cells * spec_me4.stages * args.steps / r_me4["wall_s"] / 1e6

@@ -4,7 +4,7 @@ the OUxSBLI compressible-flow solver.
 The compute path is the in-tree HIP library ``libouxsbli.so`` (C ABI of
 include/ouxsbli.h) loaded with ctypes; there is no CPU fallback.
 """
-from .cases import CaseSpec, config1_spec, config2_spec, tgv_spec, tgv_state  # noqa: F401
+from .cases import CaseSpec, tgv_keep4_spec, supersonic_tgv_spec, tgv_spec, tgv_state  # noqa: F401
 from .case import Case  # noqa: F401
 from .patcher import patch  # noqa: F401
 from .ic import UniformIC, TaylorGreenIC, RiemannIC, PeriodicBC  # noqa: F401

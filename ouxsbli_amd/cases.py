@@ -260,12 +260,12 @@ def initial_state(spec: CaseSpec) -> np.ndarray:
 
 
 # The benchmark workloads (interior sizes; extents add the ghosts).
-def config1_spec(n: int = 64) -> CaseSpec:
+def tgv_keep4_spec(n: int = 64) -> CaseSpec:
     """Plain FP64 NS TGV, KEEP ORDER 4, central-4 viscous, RK4, Ma 0.1, acoustic CFL 0.3."""
     return tgv_spec(n, 4, Ma=0.1, cfl=0.3, cfl_mode="acoustic", scheme="KEEP", visc="NS", rk=4, tvd="none")
 
 
-def config2_spec(n: int = 256, precision: str = "fp64") -> CaseSpec:
+def supersonic_tgv_spec(n: int = 256, precision: str = "fp64") -> CaseSpec:
     """Supersonic NS TGV, Ma 1.25, SLAU ORDER 6 HR-SLAU2, hybrid TVD (threshold 0.4),
     central-6 viscous, RK3-TVD, CFL 0.03 (the NSTGV case file)."""
     return tgv_spec(n, 6, Ma=1.25, cfl=0.03, scheme="SLAU", visc="NS", rk=3, tvd="hybrid",

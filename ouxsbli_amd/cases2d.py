@@ -246,7 +246,7 @@ CFL_REF = DT_REF * (free_stream()[0] + math.sqrt(GAMMA * R_GAS * free_stream()[2
                                                                                       LY / (NY_REF - 1))
 
 
-def config4_specs(ncase: int = 64, seed: int = 20261206, precision: str = "fp32_bulk", scheme: str = "Roe",
+def oblique_shock_specs(ncase: int = 64, seed: int = 20261206, precision: str = "fp32_bulk", scheme: str = "Roe",
                   order: int = 6, nx_range=(128, 512), ny_range=(64, 256), theta_deg=(4.0, 12.0)) -> List[CaseSpec2D]:
     """The batched oblique-shock ensemble: interior sizes nx-2 ~ U{128..512}
     and ny-2 ~ U{64..256} drawn independently, deflection theta ~ U(4, 12)

@@ -76,7 +76,7 @@ def khi_stream(n_requests: int, rate_hz: float = 20.0, seed: int = 20261207, n_m
 def dhit_stream(n_requests: int, rate_hz: float = 4.0, seed: int = 20261208, n_min: int = 64, n_max: int = 192,
                 multiple: int = 32, nsteps: int = 50, precision: str = "fp32_bulk",
                 with_states: bool = False) -> List[Request]:
-    """An LES request stream: decaying isotropic turbulence (config 3's
+    """An LES request stream: decaying isotropic turbulence (the LES
     scheme: Hybrid KEEP/SLAU ORDER 6, SMS model, RK4) on interiors uniform
     over the multiples of ``multiple`` in [n_min, n_max], each with its own
     spectral seed, exponential inter-arrival times (mean 1/rate_hz).  By

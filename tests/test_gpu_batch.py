@@ -156,7 +156,7 @@ def test_les_alone_equals_in_mix(precision, use_graph):
 
 
 def test_batched_les_fp64_equals_oracle():
-    """A batched FP64 LES case (config 3's scheme on its turbulent field)
+    """A batched FP64 LES case (the LES isotropic-turbulence scheme on its field)
     equals the oracle bit for bit; 72^3 gives two segments per line sweep
     at the default segment length and two tiles of the SGS launch along x."""
     spec = cases.dhit_spec(72, "fp64", seed=6)

@@ -25,7 +25,7 @@ CASES = {
     "slau6-muscl-y-z-slip": ((60, 72, 40), 6, "SLAU", "tvd", "HRSLAU2", 3, ("extrapolate", "slip", "slip"), 3),
     "keep4-rk4-walls": ((20, 24, 28), 4, "KEEP", "none", "HRSLAU2", 4, ("slip", "slip", "slip"), 4),
     "slau4-y-open": ((36, 30, 20), 4, "SLAU", "tvd", "SLAU", 3, ("periodic", "extrapolate", "periodic"), 4),
-    # Navier-Stokes: config 2's scheme between slip walls in y and z, and
+    # Navier-Stokes: the supersonic TGV's scheme between slip walls in y and z, and
     # ME4Base / central-4 variants
     "slau6-ns6-y-z-slip": ((130, 40, 72), 6, "SLAU", "hybrid", "HRSLAU2", 3, ("periodic", "slip", "slip"), 2,
                            "NS", 6, "Central"),

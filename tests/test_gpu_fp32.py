@@ -27,7 +27,7 @@ KE_REL_TOL = 1e-5
 # no switch and stays at rounding level.
 CASES = {
     # name: (spec builder, steps, field bound)
-    "slau6-ns6-hybrid-rk3-ma1.25": (lambda p: cases.config2_spec(32, p), 40, 1e-3),
+    "slau6-ns6-hybrid-rk3-ma1.25": (lambda p: cases.supersonic_tgv_spec(32, p), 40, 1e-3),
     "keep4-ns4-rk4-ma0.1": (lambda p: cases.tgv_spec(32, 4, Ma=0.1, cfl=0.3, cfl_mode="acoustic", scheme="KEEP",
                                                       visc="NS", rk=4, tvd="none", precision=p), 40, 1e-5),
     "hybrid6-les6-rk4-ma0.8": (lambda p: cases.tgv_spec(24, 6, Ma=0.8, cfl=0.3, cfl_mode="acoustic",

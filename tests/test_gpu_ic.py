@@ -46,9 +46,9 @@ def _close(a, b, tol):
             assert np.abs(a[:, :, l, :] - b[:, :, l, :]).max() <= tol * scale, f"variable {l}"
 
 
-@pytest.mark.parametrize("spec", [cases.config2_spec(40), cases.config1_spec(24),
+@pytest.mark.parametrize("spec", [cases.supersonic_tgv_spec(40), cases.tgv_keep4_spec(24),
                                   cases.tgv_spec(30, 4, Ma=0.3, scheme="KEEP", visc="Euler", rk=3)],
-                         ids=["config2-40", "config1-24", "euler-30"])
+                         ids=["supersonic-tgv-40", "tgv-keep4-24", "euler-30"])
 def test_taylor_green_matches_host(spec):
     print(f"device TGV {spec.interior} ORDER {spec.order}", flush=True)
     got, _ = _device_state(spec)
@@ -79,7 +79,7 @@ def test_oblique_shock_matches_host_bitwise():
     assert np.array_equal(got, up)
 
 
-@pytest.mark.parametrize("spec", [cases.config1_spec(16), cases.khi_spec(24, 6, "KEEP", seed=5)],
+@pytest.mark.parametrize("spec", [cases.tgv_keep4_spec(16), cases.khi_spec(24, 6, "KEEP", seed=5)],
                          ids=["rk4-keep4-ns", "rk3-keep6-euler"])
 def test_init_state_leaves_the_handle_as_an_upload_does(spec):
     """Run A from ox_init_state, run B from ox_set_qj of A's initial QJ, both
@@ -140,7 +140,7 @@ def test_device_ic_is_refused_without_parameters():
 
 @pytest.mark.parametrize("n", [24, 64, 96])
 def test_isotropic_spectrum_matches_host_form(n):
-    """Config 3's isotropic field synthesised on the device from its mode box
+    """The LES case's isotropic field synthesised on the device from its mode box
     (OX_IC_SPECTRAL) against the host form (inverse real FFT of the whole half
     spectrum, cases.dhit_state): within 1e-12 of each field's scale.  At
     n = 64 and 96 the box (|k| <= 28) leaves out modes the FFT keeps; their

@@ -103,10 +103,10 @@ def test_steps_bitwise(case):
     np.testing.assert_array_equal(QJg, o.get_qj())
 
 
-def test_config1_100_steps():
-    """Config 1 (64^3 KEEP4 NS RK4, Ma 0.1): all five conserved fields after
+def test_tgv64_keep4_100_steps():
+    """The NS TGV at 64^3 (KEEP4, central-4 viscous, RK4, Ma 0.1): all five conserved fields after
     100 steps within 1e-12 relative of the oracle (expected: identical)."""
-    spec = cases.config1_spec(64)
+    spec = cases.tgv_keep4_spec(64)
     Q = cases.tgv_state(spec)
     o = Oracle(spec)
     o.set_state(Q)
@@ -157,7 +157,7 @@ def test_interleaved_calls_bitwise(case):
 
 
 def test_les_dhit_bitwise():
-    """Config 3's scheme (Hybrid KEEP/SLAU with the Ducros switch, SMS LES,
+    """The LES scheme (Hybrid KEEP/SLAU with the Ducros switch, SMS LES,
     RK4) on its turbulent initial field at 24^3: the GPU FP64 form equals the
     oracle after 3 steps, bit for bit (the LES selector and the hybrid switch
     are both active on this field)."""

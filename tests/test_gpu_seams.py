@@ -25,11 +25,11 @@ pytestmark = pytest.mark.gpu
 SEAM_N = (130, 72, 136)
 
 SCHEMES = {
-    # config 2's scheme: SLAU ORDER 6 HR-SLAU2, hybrid TVD, central-6 viscous, RK3
+    # the supersonic TGV's scheme: SLAU ORDER 6 HR-SLAU2, hybrid TVD, central-6 viscous, RK3
     "slau6-ns6-rk3": (6, "SLAU", "NS", 6, "hybrid", "HRSLAU2", 3),
     # the ME4Base viscous stencil with the same convective scheme
     "slau6-me4-rk3": (6, "SLAU", "NS", 6, "hybrid", "HRSLAU2", 3, "ME4Base"),
-    # config 1's scheme at ORDER 4 (60-cell x tiles)
+    # the KEEP4 NS TGV's scheme at ORDER 4 (60-cell x tiles)
     "keep4-ns4-rk4": (4, "KEEP", "NS", 4, "none", "HRSLAU2", 4),
 }
 
@@ -133,7 +133,7 @@ def _numpy_phase_field(spec):
 
 @pytest.mark.parametrize("field", ["counter-phases", "numpy-phases"])
 def test_seam_les_dhit_bitwise(field):
-    """Config 3's scheme (Hybrid KEEP/SLAU ORDER 6, SMS LES, RK4) on its
+    """The LES scheme (Hybrid KEEP/SLAU ORDER 6, SMS LES, RK4) on its
     turbulent field at 124^3: three x tiles and two segments per line sweep."""
     spec = cases.dhit_spec(124, "fp64", seed=5)
     Q = cases.initial_state(spec) if field == "counter-phases" else _numpy_phase_field(spec)

@@ -175,7 +175,7 @@ def test_parameters_follow_fortran_integer_semantics():
 
 def test_reference_import_names():
     """The reference's scripts import from ``ouxsbli`` (tutorials/test.py:3-5,
-    ouxsbli/tests/test_st.py:10); those names resolve to this build."""
+    ouxsbli/tests/test_st.py:10); those names resolve to the ouxsbli_amd package."""
     import ouxsbli
     from ouxsbli import Case as C2
     from ouxsbli.patcher import patch as p2

@@ -73,8 +73,9 @@ def test_walls_bitwise_vs_oracle(name):
     np.testing.assert_array_equal(Rg2[sl], o.rhs()[sl])
 
 
-def test_walls_fp32_bulk_close_to_oracle():
-    spec, Q, steps = _case("hybrid6-x-slip-z-open", "fp32_bulk")
+@pytest.mark.parametrize("name", ["hybrid6-x-slip-z-open", "slau6-ns6-y-z-slip"])
+def test_walls_fp32_bulk_close_to_oracle(name):
+    spec, Q, steps = _case(name, "fp32_bulk")
     o = Oracle(CaseSpec(**{**spec.__dict__, "precision": "fp64"}))
     o.set_state(Q)
     o.step(steps)

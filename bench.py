@@ -43,18 +43,20 @@ def kernel_model(spec, ops):
     padded = spec.nx * spec.ny * spec.nz
     cells = spec.interior_cells
     e = 8 if spec.precision == "fp64" else 4
+    np_ = 5 if spec.precision == "fp64" else 7
     face = ops["conv_face_smooth"] + 2 + ops["visc_face"] + 4 * ops["cross_deriv"]
     m = {
         # stale primitives only (after a host upload)
         "prim": {"bytes": padded * (40 + 7 * e), "ops": padded * 20},
-        # reads rho,u,v,w,p,T,mu; writes the sensor and the residual
-        "sweep_x": {"bytes": cells * (7 * e + e + 5 * e), "ops": cells * (face + 10)},
-        # reads the 8 primitives; reads and writes the residual
-        "sweep_y": {"bytes": cells * (8 * e + 10 * e), "ops": cells * (face + 15)},
-        # reads the 8 primitives and the residual; RK3 average of the state
-        # traffic (stage 1: Qin read, Qout write; stages 2-3: also Qout read);
-        # writes the 7 primitives of the new state
-        "sweep_z_rk": {"bytes": cells * (8 * e + 5 * e + 40 + 40 + 40 * 2 / 3 + 7 * e),
+        # primitives stored per cell: rho,u,v,w,p (+T,mu in FP32 storage; the
+        # FP64 sweeps form T and mu from rho and p)
+        "sweep_x": {"bytes": cells * (np_ * e + e + 5 * e), "ops": cells * (face + 10)},
+        # reads the primitives and the sensor; reads and writes the residual
+        "sweep_y": {"bytes": cells * ((np_ + 1) * e + 10 * e), "ops": cells * (face + 15)},
+        # reads the primitives, the sensor and the residual; RK3 average of
+        # the state traffic (stage 1: Qin read, Qout write; stages 2-3: also
+        # Qout read); writes the primitives of the new state
+        "sweep_z_rk": {"bytes": cells * ((np_ + 1) * e + 5 * e + 40 + 40 + 40 * 2 / 3 + np_ * e),
                        "ops": cells * (face + 15 + 4 * 5 + 20)},
         "ghost_fill": {"bytes": (padded - cells) * 80, "ops": 0},
     }

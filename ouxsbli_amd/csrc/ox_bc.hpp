@@ -79,8 +79,10 @@ __global__ __launch_bounds__(256) void k_bc_prim(Prim<S> P, Geom g, int d) {
   G(P.v)[m] = (flip && d == 1) ? -v : v;
   G(P.w)[m] = (flip && d == 2) ? -w : w;
   G(P.p)[m] = G(P.p)[sm];
-  G(P.T)[m] = G(P.T)[sm];
-  if constexpr (VISC) G(P.mu)[m] = G(P.mu)[sm];
+  if constexpr (!kRecomputeTMu<S>) {
+    G(P.T)[m] = G(P.T)[sm];
+    if constexpr (VISC) G(P.mu)[m] = G(P.mu)[sm];
+  }
 }
 
 // The Ducros sensor of the ghost layer next to the interior along a

@@ -212,7 +212,8 @@ void rk_stage(ox_solver* s, int st, const double** Qsrc, ox::StageArgs* sa, doub
 size_t solver_footprint(const ox_config& c) {
   const size_t ns = (size_t)c.nx * 5 * c.ny * c.nz, nc = (size_t)c.nx * c.ny * c.nz;
   const size_t pe = c.precision == OX_PREC_FP64 ? sizeof(double) : sizeof(float);
-  const size_t nprim = c.visc == OX_VISC_LES ? 10 : 8;
+  // rho,u,v,w,p,sensor (+T,mu in FP32 storage; FP64 sweeps form them) (+mut,qc2 for LES)
+  const size_t nprim = 6 + (c.precision == OX_PREC_FP64 ? 0 : 2) + (c.visc == OX_VISC_LES ? 2 : 0);
   size_t b = 2 * ns * sizeof(double) + ns * pe + 2 * nprim * nc * pe;  // QJ, QJ2, Racc, two primitive sets
   if (c.rk == 4) b += ns * sizeof(double);                             // Rs
   b += 4096 * (2 * ox::DIAG_NQ + 1) * sizeof(double);                  // reduction partials
@@ -400,9 +401,12 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   ok = ok && hipMalloc(&s->QJ2, ns * sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc(&C.Racc, ns * pe) == hipSuccess;
   if (cfg->rk == 4) ok = ok && hipMalloc(&s->Rs, ns * sizeof(double)) == hipSuccess;
+  // arrays 5 and 6 (T, mu) only for FP32 storage; 8 and 9 (mut, qc2) only for LES
   const int nprim = cfg->visc == OX_VISC_LES ? 10 : 8;
+  auto used = [&](int m) { return !((m == 5 || m == 6) && cfg->precision == OX_PREC_FP64); };
   for (int set = 0; set < 2; ++set)
-    for (int m = 0; m < nprim && ok; ++m) ok = hipMalloc(&C.prim[set][m], nc * pe) == hipSuccess;
+    for (int m = 0; m < nprim && ok; ++m)
+      if (used(m)) ok = hipMalloc(&C.prim[set][m], nc * pe) == hipSuccess;
   s->scratch_bytes = 4096 * (2 * ox::DIAG_NQ + 1) * sizeof(double);
   ok = ok && hipMalloc(&s->scratch, s->scratch_bytes) == hipSuccess;
   ok = ok && hipEventCreate(&s->ev0) == hipSuccess && hipEventCreate(&s->ev1) == hipSuccess;
@@ -416,7 +420,8 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   ok = ok && hipMemsetAsync(C.Racc, 0, ns * pe, C.stream) == hipSuccess;
   if (s->Rs) ok = ok && hipMemsetAsync(s->Rs, 0, ns * sizeof(double), C.stream) == hipSuccess;
   for (int set = 0; set < 2; ++set)
-    for (int m = 0; m < nprim && ok; ++m) ok = hipMemsetAsync(C.prim[set][m], 0, nc * pe, C.stream) == hipSuccess;
+    for (int m = 0; m < nprim && ok; ++m)
+      if (used(m)) ok = hipMemsetAsync(C.prim[set][m], 0, nc * pe, C.stream) == hipSuccess;
   ok = ok && hipStreamSynchronize(C.stream) == hipSuccess;
   if (!ok) {
     destroy(s);

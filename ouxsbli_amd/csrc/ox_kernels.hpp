@@ -92,6 +92,13 @@ __device__ __forceinline__ PrimVals prim_of(double q1, double q2, double q3, dou
   if constexpr (VISC) r.mu = c.mu0c / (r.T + 111.0) * (r.T * fsqrt(r.T));
   return r;
 }
+// FP64 storage: the stage epilogue does not store T and mu; the sweeps form
+// them from the stored rho and p with prim_of's own expressions, the same
+// values bit for bit.  That takes 16 B per cell off the HBM-bound z sweep's
+// writes and 16 B off each sweep's reads, for one division, one square root
+// and one more division per window cell.  (FP32 storage keeps T and mu: they
+// are recovered in FP64 from the state, which rho and p in FP32 cannot repeat.)
+template <class S> constexpr bool kRecomputeTMu = sizeof(S) == 8;
 template <class S, bool VISC>
 __device__ __forceinline__ void store_prim(const Prim<S>& P, int m, const PrimVals& r) {
   G(P.rho)[m] = (S)r.rho;
@@ -99,8 +106,10 @@ __device__ __forceinline__ void store_prim(const Prim<S>& P, int m, const PrimVa
   G(P.v)[m] = (S)r.v;
   G(P.w)[m] = (S)r.w;
   G(P.p)[m] = (S)r.p;
-  G(P.T)[m] = (S)r.T;
-  if constexpr (VISC) G(P.mu)[m] = (S)r.mu;
+  if constexpr (!kRecomputeTMu<S>) {
+    G(P.T)[m] = (S)r.T;
+    if constexpr (VISC) G(P.mu)[m] = (S)r.mu;
+  }
 }
 
 template <class S, bool VISC>
@@ -305,10 +314,10 @@ __device__ __forceinline__ void fetch_cell(const Prim<S>& P, const Geom& g, int 
   r.v = G(P.v)[m];
   r.w = G(P.w)[m];
   r.p = G(P.p)[m];
-  if constexpr (kUsesT<SCHEME> || VISC) r.T = G(P.T)[m];
+  if constexpr ((kUsesT<SCHEME> || VISC) && !kRecomputeTMu<S>) r.T = G(P.T)[m];
   if constexpr (LOAD_S && kUsesSensor<SCHEME, ORDER>) r.s = G(P.s)[ms];
   if constexpr (VISC) {
-    r.mu = G(P.mu)[m];
+    if constexpr (!kRecomputeTMu<S>) r.mu = G(P.mu)[m];
     if constexpr (VISC == 2) {
       r.mt = G(P.mut)[m];
       r.qc2 = G(P.qc2)[m];
@@ -348,10 +357,16 @@ __device__ __forceinline__ void finish_cell(const Geom& g, const PhysC<R>& c, co
   d.v = R(r.v);
   d.w = R(r.w);
   d.p = R(r.p);
-  if constexpr (kUsesT<SCHEME> || VISC) d.T = R(r.T);
+  if constexpr (kRecomputeTMu<S>) {
+    // prim_of: T = p / (R rho), Sutherland mu
+    if constexpr (kUsesT<SCHEME> || VISC) d.T = d.p / (c.Rgas * d.rho);
+    if constexpr (VISC) d.mu = c.mu0c / (d.T + 111.0) * (d.T * fsqrt(d.T));
+  } else {
+    if constexpr (kUsesT<SCHEME> || VISC) d.T = R(r.T);
+    if constexpr (VISC) d.mu = R(r.mu);
+  }
   if constexpr (LOAD_S && kUsesSensor<SCHEME, ORDER>) d.s = R(r.s);
   if constexpr (VISC) {
-    d.mu = R(r.mu);
     if constexpr (VISC == 2) {
       d.mt = R(r.mt);
       d.H = c.Cp * d.T + R(0.5) * (d.u * d.u + d.v * d.v + d.w * d.w) + R(r.qc2);

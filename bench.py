@@ -43,13 +43,11 @@ def kernel_model(spec, ops):
     padded = spec.nx * spec.ny * spec.nz
     cells = spec.interior_cells
     e = 8 if spec.precision == "fp64" else 4
-    np_ = 5 if spec.precision == "fp64" else 7
+    np_ = 5  # rho, u, v, w, p: the sweeps form T and mu from rho and p
     face = ops["conv_face_smooth"] + 2 + ops["visc_face"] + 4 * ops["cross_deriv"]
     m = {
         # stale primitives only (after a host upload)
-        "prim": {"bytes": padded * (40 + 7 * e), "ops": padded * 20},
-        # primitives stored per cell: rho,u,v,w,p (+T,mu in FP32 storage; the
-        # FP64 sweeps form T and mu from rho and p)
+        "prim": {"bytes": padded * (40 + np_ * e), "ops": padded * 20},
         "sweep_x": {"bytes": cells * (np_ * e + e + 5 * e), "ops": cells * (face + 10)},
         # reads the primitives and the sensor; reads and writes the residual
         "sweep_y": {"bytes": cells * ((np_ + 1) * e + 10 * e), "ops": cells * (face + 15)},

@@ -104,7 +104,7 @@ struct ox_batch {
   struct BcFill {
     size_t li;
     ox::Geom g;
-    void* pn[10];  // the primitive set that launch's epilogue writes
+    void* pn[8];  // the primitive set that launch's epilogue writes
   };
   std::vector<BcFill> bc_fills[2];
   size_t args_bytes = 0;     // one descriptor set (all launches of one step)
@@ -370,7 +370,7 @@ int build_plan(ox_batch* b) {
           ox_batch::BcFill f;
           f.li = li;
           f.g = g;
-          for (int m2 = 0; m2 < 10; ++m2) f.pn[m2] = s->C.prim[1 - s->C.cur][m2];
+          for (int m2 = 0; m2 < 8; ++m2) f.pn[m2] = s->C.prim[1 - s->C.cur][m2];
           b->bc_fills[set].push_back(f);
         }
         ox::finish_stage(s->C, sa);

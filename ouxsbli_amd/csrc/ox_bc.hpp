@@ -63,7 +63,7 @@ inline long long bc_cells(const Geom& g, int d) {
 }
 
 // The primitives' ghost layers of direction d.
-template <class S, bool VISC>
+template <class S>
 __global__ __launch_bounds__(256) void k_bc_prim(Prim<S> P, Geom g, int d) {
   int i, j, k;
   if (!bc_cell(g, d, (long long)blockIdx.x * 256 + threadIdx.x, i, j, k)) return;
@@ -79,10 +79,6 @@ __global__ __launch_bounds__(256) void k_bc_prim(Prim<S> P, Geom g, int d) {
   G(P.v)[m] = (flip && d == 1) ? -v : v;
   G(P.w)[m] = (flip && d == 2) ? -w : w;
   G(P.p)[m] = G(P.p)[sm];
-  if constexpr (!kRecomputeTMu<S>) {
-    G(P.T)[m] = G(P.T)[sm];
-    if constexpr (VISC) G(P.mu)[m] = G(P.mu)[sm];
-  }
 }
 
 // The Ducros sensor of the ghost layer next to the interior along a
@@ -125,7 +121,7 @@ inline bool any_bc(const Geom& g) { return g.bc[0] || g.bc[1] || g.bc[2]; }
 template <class R, class S, int ORDER, int SCHEME, int VISC>
 void launch_bc_prims(const Geom& g, const Prim<S>& Pn, hipStream_t st) {
   for (int d = 0; d < 3; ++d)
-    if (g.bc[d]) k_bc_prim<S, (VISC != 0)><<<(unsigned)((bc_cells(g, d) + 255) / 256), 256, 0, st>>>(Pn, g, d);
+    if (g.bc[d]) k_bc_prim<S><<<(unsigned)((bc_cells(g, d) + 255) / 256), 256, 0, st>>>(Pn, g, d);
 }
 template <class R, class S, int ORDER, int SCHEME>
 void launch_bc_sensors(const Geom& g, const Prim<S>& P, hipStream_t st) {

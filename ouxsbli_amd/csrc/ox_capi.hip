@@ -212,8 +212,8 @@ void rk_stage(ox_solver* s, int st, const double** Qsrc, ox::StageArgs* sa, doub
 size_t solver_footprint(const ox_config& c) {
   const size_t ns = (size_t)c.nx * 5 * c.ny * c.nz, nc = (size_t)c.nx * c.ny * c.nz;
   const size_t pe = c.precision == OX_PREC_FP64 ? sizeof(double) : sizeof(float);
-  // rho,u,v,w,p,sensor (+T,mu in FP32 storage; FP64 sweeps form them) (+mut,qc2 for LES)
-  const size_t nprim = 6 + (c.precision == OX_PREC_FP64 ? 0 : 2) + (c.visc == OX_VISC_LES ? 2 : 0);
+  // rho,u,v,w,p,sensor (+mut,qc2 for LES); the sweeps form T and mu
+  const size_t nprim = 6 + (c.visc == OX_VISC_LES ? 2 : 0);
   size_t b = 2 * ns * sizeof(double) + ns * pe + 2 * nprim * nc * pe;  // QJ, QJ2, Racc, two primitive sets
   if (c.rk == 4) b += ns * sizeof(double);                             // Rs
   b += 4096 * (2 * ox::DIAG_NQ + 1) * sizeof(double);                  // reduction partials
@@ -281,7 +281,7 @@ void destroy(ox_solver* s) {
   hipFree(s->scratch);
   hipFree(s->C.Racc);
   for (int set = 0; set < 2; ++set)
-    for (int m = 0; m < 10; ++m) hipFree(s->C.prim[set][m]);
+    for (int m = 0; m < ox::kPrimArrays; ++m) hipFree(s->C.prim[set][m]);
   for (hipEvent_t e : s->prof.pool) hipEventDestroy(e);
   if (s->ev0) hipEventDestroy(s->ev0);
   if (s->ev1) hipEventDestroy(s->ev1);
@@ -365,7 +365,7 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   C.variant = cfg->slau_variant;
   C.Racc = nullptr;
   for (int set = 0; set < 2; ++set)
-    for (int m = 0; m < 10; ++m) C.prim[set][m] = nullptr;
+    for (int m = 0; m < ox::kPrimArrays; ++m) C.prim[set][m] = nullptr;
   C.cur = 0;
   C.prim_for = nullptr;
   C.stream = nullptr;
@@ -401,12 +401,11 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   ok = ok && hipMalloc(&s->QJ2, ns * sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc(&C.Racc, ns * pe) == hipSuccess;
   if (cfg->rk == 4) ok = ok && hipMalloc(&s->Rs, ns * sizeof(double)) == hipSuccess;
-  // arrays 5 and 6 (T, mu) only for FP32 storage; 8 and 9 (mut, qc2) only for LES
-  const int nprim = cfg->visc == OX_VISC_LES ? 10 : 8;
-  auto used = [&](int m) { return !((m == 5 || m == 6) && cfg->precision == OX_PREC_FP64); };
+  // arrays 6 and 7 (mut, qc2) only for LES
+  const int nprim = cfg->visc == OX_VISC_LES ? 8 : 6;
   for (int set = 0; set < 2; ++set)
     for (int m = 0; m < nprim && ok; ++m)
-      if (used(m)) ok = hipMalloc(&C.prim[set][m], nc * pe) == hipSuccess;
+      ok = hipMalloc(&C.prim[set][m], nc * pe) == hipSuccess;
   s->scratch_bytes = 4096 * (2 * ox::DIAG_NQ + 1) * sizeof(double);
   ok = ok && hipMalloc(&s->scratch, s->scratch_bytes) == hipSuccess;
   ok = ok && hipEventCreate(&s->ev0) == hipSuccess && hipEventCreate(&s->ev1) == hipSuccess;
@@ -421,7 +420,7 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   if (s->Rs) ok = ok && hipMemsetAsync(s->Rs, 0, ns * sizeof(double), C.stream) == hipSuccess;
   for (int set = 0; set < 2; ++set)
     for (int m = 0; m < nprim && ok; ++m)
-      if (used(m)) ok = hipMemsetAsync(C.prim[set][m], 0, nc * pe, C.stream) == hipSuccess;
+      ok = hipMemsetAsync(C.prim[set][m], 0, nc * pe, C.stream) == hipSuccess;
   ok = ok && hipStreamSynchronize(C.stream) == hipSuccess;
   if (!ok) {
     destroy(s);

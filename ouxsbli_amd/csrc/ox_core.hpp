@@ -16,11 +16,11 @@ struct Core {
   int vorder;        // viscous order actually used (0 for Euler)
   int tvd, variant;  // runtime switches (uniform across the launch)
   // Two sets of primitive arrays (element type per precision mode):
-  // rho,u,v,w,p,T,mu,sensor,mut,qc2.  A stage reads set `cur`; its RK
+  // rho,u,v,w,p,sensor,mut,qc2 (mut, qc2 for LES only).  A stage reads set `cur`; its RK
   // epilogue writes the primitives of the stage's new state into the other
   // set, which becomes current.  `prim_for` is the state buffer the current
   // set was computed from (null: stale, k_prim runs first).
-  void* prim[2][10];
+  void* prim[2][8];
   int cur;
   const double* prim_for;
   // residual accumulator between the sweeps, state layout, element type S

@@ -4,15 +4,16 @@
 // Data layout in HBM (all on one GPU, ghosts included):
 //   state  QJ(nx,5,ny,nz) FP64, the reference layout (x fastest, then the
 //          variable, then y, z): drop-in for set_state/get_state/checkpoints.
-//   prims  rho,u,v,w,p,T,mu,sensor as separate (nx,ny,nz) arrays of type S
-//          (FP64 plain form, FP32 in the FP32-bulk form).
-//   Racc   residual accumulator in the state layout, FP64.
+//   prims  rho,u,v,w,p,sensor as separate (nx,ny,nz) arrays of type S
+//          (FP64 plain form, FP32 in the FP32-bulk form); T and mu are
+//          formed from rho and p where a sweep reads a cell.
+//   Racc   residual accumulator in the state layout, type S.
 //
-// One stage = k_prim (QJ -> primitives, whole grid) ->
-// k_sweep_x (LDS row tiles; Ducros sensor; writes R = c_x*dE) -> k_sweep_line<DIR=1>
+// One stage = k_sweep_x (LDS row tiles; Ducros sensor; writes R = c_x*dE) -> k_sweep_line<DIR=1>
 // (register sliding window along y; R += c_y*dF) -> k_sweep_line<DIR=2>
-// (sliding window along z; R += c_z*dG, then the RK combination) ->
-// k_ghost_fill.  Face fluxes (convective minus viscous) live only in
+// (sliding window along z; R += c_z*dG, then the RK combination, and the
+// primitives of the new state for the next stage).  k_prim (QJ -> primitives)
+// runs only after the state is uploaded, k_ghost_fill at the end of a call.  Face fluxes (convective minus viscous) live only in
 // registers: E, F and G are never written to memory.  The summation order
 // x, then y, then z is the order of calc_R (3D_solver/src/calc_steps.f90:8-38).
 #pragma once
@@ -25,8 +26,9 @@
 namespace ox {
 
 template <class S> struct Prim {
-  S *rho, *u, *v, *w, *p, *T, *mu, *s, *mut, *qc2;
+  S *rho, *u, *v, *w, *p, *s, *mut, *qc2;
 };
+constexpr int kPrimArrays = 8;
 
 // The pointer seen as one into global memory.  Pointers a kernel receives as
 // arguments are known global; those the batched kernels read from their
@@ -76,9 +78,8 @@ __device__ __forceinline__ int cidx(const Geom& g, int i, int j, int k) { return
 // ---------------------------------------------------------------------------
 // Primitives of one cell from its conserved values (J * Q = physical).
 struct PrimVals {
-  double rho, u, v, w, p, T, mu;
+  double rho, u, v, w, p;
 };
-template <bool VISC>
 __device__ __forceinline__ PrimVals prim_of(double q1, double q2, double q3, double q4, double q5, double J,
                                             const PhysC<double>& c) {
   PrimVals r;
@@ -88,31 +89,24 @@ __device__ __forceinline__ PrimVals prim_of(double q1, double q2, double q3, dou
   r.v = q3 * over_Q1;
   r.w = q4 * over_Q1;
   r.p = c.gamma_1 * (J * q5 - 0.5 * r.rho * (r.u * r.u + r.v * r.v + r.w * r.w));
-  r.T = r.p / (c.Rgas * r.rho);
-  if constexpr (VISC) r.mu = c.mu0c / (r.T + 111.0) * (r.T * fsqrt(r.T));
   return r;
 }
-// FP64 storage: the stage epilogue does not store T and mu; the sweeps form
-// them from the stored rho and p with prim_of's own expressions, the same
-// values bit for bit.  That takes 16 B per cell off the HBM-bound z sweep's
-// writes and 16 B off each sweep's reads, for one division, one square root
-// and one more division per window cell.  (FP32 storage keeps T and mu: they
-// are recovered in FP64 from the state, which rho and p in FP32 cannot repeat.)
-template <class S> constexpr bool kRecomputeTMu = sizeof(S) == 8;
-template <class S, bool VISC>
+// T and mu are not stored: the sweeps form them from the stored rho and p
+// where they read a cell (finish_cell), with the reference's expressions
+// (calc_quantities_T_3D, src/calc_physical_quantities.f90:60-116), the same
+// values bit for bit as forming them here in FP64.  That takes two arrays off
+// the z sweep's writes and off every sweep's reads, for one division, one
+// square root and one more division per window cell.
+template <class S>
 __device__ __forceinline__ void store_prim(const Prim<S>& P, int m, const PrimVals& r) {
   G(P.rho)[m] = (S)r.rho;
   G(P.u)[m] = (S)r.u;
   G(P.v)[m] = (S)r.v;
   G(P.w)[m] = (S)r.w;
   G(P.p)[m] = (S)r.p;
-  if constexpr (!kRecomputeTMu<S>) {
-    G(P.T)[m] = (S)r.T;
-    if constexpr (VISC) G(P.mu)[m] = (S)r.mu;
-  }
 }
 
-template <class S, bool VISC>
+template <class S>
 __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ QJ, Prim<S> P, Geom g, PhysC<double> c) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int j = blockIdx.y * 4 + threadIdx.y;
@@ -120,8 +114,8 @@ __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ QJ, Pri
   if (i >= g.nx || j >= g.ny) return;
   const int q = qidx(g, i, 0, j, k);
   const int st = g.nx;
-  const PrimVals r = prim_of<VISC>(QJ[q], QJ[q + st], QJ[q + 2 * st], QJ[q + 3 * st], QJ[q + 4 * st], g.jac, c);
-  store_prim<S, VISC>(P, cidx(g, i, j, k), r);
+  const PrimVals r = prim_of(QJ[q], QJ[q + st], QJ[q + 2 * st], QJ[q + 3 * st], QJ[q + 4 * st], g.jac, c);
+  store_prim<S>(P, cidx(g, i, j, k), r);
 }
 
 // ---------------------------------------------------------------------------
@@ -288,7 +282,7 @@ template <class R, int DIR, int ORDER, int SCHEME, int VISC, int VO> struct Cell
 // one face ahead of their use.
 template <class S, int VISC, int VO> struct RawCell {
   static constexpr int NX = VISC ? vcross<VO>() : 1;
-  S rho, u, v, w, p, T, mu, s, mt, qc2;
+  S rho, u, v, w, p, s, mt, qc2;
   S x[4][NX];
 };
 
@@ -314,10 +308,8 @@ __device__ __forceinline__ void fetch_cell(const Prim<S>& P, const Geom& g, int 
   r.v = G(P.v)[m];
   r.w = G(P.w)[m];
   r.p = G(P.p)[m];
-  if constexpr ((kUsesT<SCHEME> || VISC) && !kRecomputeTMu<S>) r.T = G(P.T)[m];
   if constexpr (LOAD_S && kUsesSensor<SCHEME, ORDER>) r.s = G(P.s)[ms];
   if constexpr (VISC) {
-    if constexpr (!kRecomputeTMu<S>) r.mu = G(P.mu)[m];
     if constexpr (VISC == 2) {
       r.mt = G(P.mut)[m];
       r.qc2 = G(P.qc2)[m];
@@ -357,14 +349,10 @@ __device__ __forceinline__ void finish_cell(const Geom& g, const PhysC<R>& c, co
   d.v = R(r.v);
   d.w = R(r.w);
   d.p = R(r.p);
-  if constexpr (kRecomputeTMu<S>) {
-    // prim_of: T = p / (R rho), Sutherland mu
-    if constexpr (kUsesT<SCHEME> || VISC) d.T = d.p / (c.Rgas * d.rho);
-    if constexpr (VISC) d.mu = c.mu0c / (d.T + 111.0) * (d.T * fsqrt(d.T));
-  } else {
-    if constexpr (kUsesT<SCHEME> || VISC) d.T = R(r.T);
-    if constexpr (VISC) d.mu = R(r.mu);
-  }
+  // T = p / (R rho) and Sutherland's law (calc_quantities_T_3D,
+  // src/calc_physical_quantities.f90:60-116)
+  if constexpr (kUsesT<SCHEME> || VISC) d.T = d.p / (c.Rgas * d.rho);
+  if constexpr (VISC) d.mu = c.mu0c / (d.T + R(111.0)) * (d.T * fsqrt(d.T));
   if constexpr (LOAD_S && kUsesSensor<SCHEME, ORDER>) d.s = R(r.s);
   if constexpr (VISC) {
     if constexpr (VISC == 2) {
@@ -946,9 +934,9 @@ __device__ __forceinline__ void sweep_line_tile(const SweepArgs<R, S>& a, int bx
           // would compute from it), into the other primitive set, at the
           // cell and at every ghost cell that is its periodic image (the
           // ghost fill of set_bc_cyclic applied to the primitives)
-          const PrimVals pv = prim_of<(VISC != 0)>(Qn[0], Qn[1], Qn[2], Qn[3], Qn[4], g.jac, c64);
+          const PrimVals pv = prim_of(Qn[0], Qn[1], Qn[2], Qn[3], Qn[4], g.jac, c64);
           const int jj = t, kk = f;
-          store_prim<S, (VISC != 0)>(Pn, cidx(g, i, jj, kk), pv);
+          store_prim<S>(Pn, cidx(g, i, jj, kk), pv);
           const int ai = i < 2 * g.h ? i + g.ix : (i >= g.ix ? i - g.ix : -1);
           const int aj = jj < 2 * g.h ? jj + g.iy : (jj >= g.iy ? jj - g.iy : -1);
           const int ak = kk < 2 * g.h ? kk + g.iz : (kk >= g.iz ? kk - g.iz : -1);
@@ -956,7 +944,7 @@ __device__ __forceinline__ void sweep_line_tile(const SweepArgs<R, S>& a, int bx
             const int xs[2] = {i, ai}, ys[2] = {jj, aj}, zs[2] = {kk, ak};
             for (int c3 = 1; c3 < 8; ++c3) {
               const int x = xs[c3 & 1], y = ys[(c3 >> 1) & 1], z = zs[(c3 >> 2) & 1];
-              if (x >= 0 && y >= 0 && z >= 0) store_prim<S, (VISC != 0)>(Pn, cidx(g, x, y, z), pv);
+              if (x >= 0 && y >= 0 && z >= 0) store_prim<S>(Pn, cidx(g, x, y, z), pv);
             }
           }
         }

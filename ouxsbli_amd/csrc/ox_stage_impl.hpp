@@ -17,8 +17,8 @@ inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) /
 
 template <class S> Prim<S> prim_set(const Core& C, int set) {
   Prim<S> P;
-  S** a[10] = {&P.rho, &P.u, &P.v, &P.w, &P.p, &P.T, &P.mu, &P.s, &P.mut, &P.qc2};
-  for (int m = 0; m < 10; ++m) *a[m] = (S*)C.prim[set][m];
+  S** a[kPrimArrays] = {&P.rho, &P.u, &P.v, &P.w, &P.p, &P.s, &P.mut, &P.qc2};
+  for (int m = 0; m < kPrimArrays; ++m) *a[m] = (S*)C.prim[set][m];
   return P;
 }
 
@@ -43,7 +43,7 @@ void prepare_stage(Core& C, const double* Qsrc) {
   const Prim<S> P = prim_set<S>(C, C.cur);
   const dim3 blk(64, 4, 1);
   if (C.prim_for != Qsrc) {
-    k_prim<S, (VISC != 0)><<<dim3(cdiv(g.nx, 64), cdiv(g.ny, 4), g.nz), blk, 0, C.stream>>>(
+    k_prim<S><<<dim3(cdiv(g.nx, 64), cdiv(g.ny, 4), g.nz), blk, 0, C.stream>>>(
         Qsrc, P, g, make_physc<double>(C.hc));
     C.prim_for = Qsrc;
     if (any_bc(g)) launch_bc_sensors<R, S, ORDER, SCHEME>(g, P, C.stream);
@@ -152,8 +152,8 @@ void make_sweep_args_erased(Core& C, const StageArgs& sa, double cx, double cy, 
 template <class R, class S, int ORDER, int SCHEME, int VISC>
 void bc_after_erased(const Geom& g, void* const* pn, hipStream_t st) {
   Prim<S> P;
-  S** a[10] = {&P.rho, &P.u, &P.v, &P.w, &P.p, &P.T, &P.mu, &P.s, &P.mut, &P.qc2};
-  for (int m = 0; m < 10; ++m) *a[m] = (S*)pn[m];
+  S** a[kPrimArrays] = {&P.rho, &P.u, &P.v, &P.w, &P.p, &P.s, &P.mut, &P.qc2};
+  for (int m = 0; m < kPrimArrays; ++m) *a[m] = (S*)pn[m];
   launch_bc_prims<R, S, ORDER, SCHEME, VISC>(g, P, st);
   launch_bc_sensors<R, S, ORDER, SCHEME>(g, P, st);
 }

@@ -125,7 +125,9 @@ typedef struct ox_solver ox_solver;
 /* Creates a solver for one case: allocates the device state, the grid metrics
  * and scratch.  Replaces allocate_device_mem + pre_calc's metric setup
  * (3D_solver/src/preprocess.f90.fypp:21-46, :48-103) and set_grid_cyclic /
- * set_Jacobian_xy3 (src/set_coordinate.f90:99-117, :197-339). */
+ * set_Jacobian_xy3 (src/set_coordinate.f90:99-117, :197-339).  Grids up to
+ * nx*ny*nz < 2^31 cells (state indices are 64-bit; about 1100^3 fits the HBM
+ * of one MI355X in FP64 NS RK3) with 15*nx*ny < 2^31 and nz <= 65535. */
 int ox_create(const ox_config* cfg, ox_solver** out);
 
 /* Frees everything the handle owns.  Replaces the deallocate at
@@ -226,7 +228,9 @@ int ox_autotune(ox_solver* s, int32_t* lseg_y, int32_t* lseg_z);
 
 /* Sets the y- and z-sweep segment lengths (cells one thread marches along the
  * sweep direction) explicitly, as ox_autotune would after timing them: any
- * value in 1..4096.  Results do not depend on the choice; the setter exists so
+ * value in 1..4096, lowered where the state offsets within one segment would
+ * reach 2^31 ((lseg_z + 2) * 5 * nx * ny for z; for the default 64, planes of
+ * over 6.5e6 cells).  Results do not depend on the choice; the setter exists so
  * that a caller can fix a tuned value, and a test can cover segment seams
  * deterministically.  No reference counterpart (fixed launch shapes,
  * calc_flux_base.f90.fypp:44-63). */

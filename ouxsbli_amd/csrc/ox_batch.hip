@@ -409,8 +409,8 @@ int tune_case(ox_batch* b, ox_solver* s) {
                                    (int)cfg.nx, (int)cfg.ny, (int)cfg.nz);
   auto it = b->tuned.find(key);
   if (it != b->tuned.end()) {
-    s->C.lseg[0] = it->second.first;
-    s->C.lseg[1] = it->second.second;
+    s->C.lseg[0] = ox::clamp_lseg(s->C.g, 1, it->second.first);
+    s->C.lseg[1] = ox::clamp_lseg(s->C.g, 2, it->second.second);
     return OX_OK;
   }
   int32_t ly, lz;
@@ -527,8 +527,8 @@ int ox_autotune(ox_solver* s, int32_t* lseg_y, int32_t* lseg_z) {
   OX_HIP(hipEventCreate(&e0));
   OX_HIP(hipEventCreate(&e1));
   auto time_rhs = [&](int ly, int lz) -> float {
-    C.lseg[0] = ly;
-    C.lseg[1] = lz;
+    C.lseg[0] = ox::clamp_lseg(g, 1, ly);
+    C.lseg[1] = ox::clamp_lseg(g, 2, lz);
     s->stage(C, s->QJ, sa, g.a_yz, g.a_zx, g.a_xy);  // warm-up
     hipEventRecord(e0, C.stream);
     for (int r = 0; r < 3; ++r) s->stage(C, s->QJ, sa, g.a_yz, g.a_zx, g.a_xy);
@@ -552,8 +552,8 @@ int ox_autotune(ox_solver* s, int32_t* lseg_y, int32_t* lseg_z) {
     const float t = time_rhs(by, lz);
     if (t < best) best = t, bz = lz;
   }
-  C.lseg[0] = by;
-  C.lseg[1] = bz;
+  C.lseg[0] = by = ox::clamp_lseg(g, 1, by);
+  C.lseg[1] = bz = ox::clamp_lseg(g, 2, bz);
   C.prof = prof;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
@@ -567,8 +567,8 @@ int ox_set_segments(ox_solver* s, int32_t lseg_y, int32_t lseg_z) {
   if (!s) return ox_fail(OX_EINVAL, "null handle");
   if (lseg_y < 1 || lseg_y > 4096 || lseg_z < 1 || lseg_z > 4096)
     return ox_fail(OX_EINVAL, "segment lengths must be in 1..4096");
-  s->C.lseg[0] = lseg_y;
-  s->C.lseg[1] = lseg_z;
+  s->C.lseg[0] = ox::clamp_lseg(s->C.g, 1, lseg_y);
+  s->C.lseg[1] = ox::clamp_lseg(s->C.g, 2, lseg_z);
   return OX_OK;
 }
 

@@ -243,7 +243,10 @@ int validate(const ox_config& c) {
   const int h = c.order / 2;
   if (c.nx < 2 * h + 2 * c.order || c.ny < 2 * h + 2 * c.order || c.nz < 2 * h + 2 * c.order)
     return ox_fail(OX_EINVAL, "grid too small for the stencil");
-  if ((double)c.nx * 5.0 * c.ny * c.nz >= 2147483647.0) return ox_fail(OX_EINVAL, "grid too large for 32-bit indexing");
+  // cell indices are 32-bit (state indices, five values per cell, 64-bit)
+  if ((double)c.nx * c.ny * c.nz >= 2147483647.0) return ox_fail(OX_EINVAL, "grid over 2^31 cells");
+  // the z sweep's 32-bit state offsets span at least three xy planes
+  if (15.0 * c.nx * c.ny >= 2147483647.0) return ox_fail(OX_EINVAL, "xy plane over 2^31 / 15 cells");
   if (c.nz > 65535 || c.ny > 65535 * 4) return ox_fail(OX_EINVAL, "grid extent exceeds launch limits");
   if (!(c.dt > 0.0) || !(c.Lx > 0.0) || !(c.Ly > 0.0) || !(c.Lz > 0.0) || !(c.gamma > 1.0) || !(c.R > 0.0))
     return ox_fail(OX_EINVAL, "non-positive dt, box length, gamma-1 or R");
@@ -369,7 +372,8 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
   C.cur = 0;
   C.prim_for = nullptr;
   C.stream = nullptr;
-  C.lseg[0] = C.lseg[1] = ox::LSEG_DEFAULT;
+  C.lseg[0] = ox::clamp_lseg(C.g, 1, ox::LSEG_DEFAULT);
+  C.lseg[1] = ox::clamp_lseg(C.g, 2, ox::LSEG_DEFAULT);
   C.prof = false;
   C.mark = &prof_record;
   C.prof_state = &s->prof;

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(DIAG_BLOCK) void k_diag(const double* __restrict__ 
     const int i = g.h + (int)(t % g.ix);
     const int j = g.h + (int)((t / g.ix) % g.iy);
     const int k = g.h + (int)(t / ((long long)g.ix * g.iy));
-    auto Q = [&](int ii, int l, int jj, int kk) { return QJ[ii + g.nx * (l + 5 * (jj + g.ny * kk))]; };
+    auto Q = [&](int ii, int l, int jj, int kk) { return QJ[ii + (long long)g.nx * (l + 5 * (jj + g.ny * kk))]; };
     auto vel = [&](int ii, int jj, int kk, int l) { return Q(ii, l, jj, kk) / Q(ii, 0, jj, kk); };
     const double q1 = Q(i, 0, j, k);
     const double rho = g.jac * q1;

@@ -20,6 +20,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "ox_physics.hpp"
 #include "ox_types.hpp"
 
@@ -66,9 +68,11 @@ template <int ORDER, int VISC, int VO> struct Sten {
   static constexpr int VH = vcross<VO>() / 2;  // half width of the cross-derivative stencil
 };
 
-// state index of (i,l,j,k), 0-based
-__device__ __forceinline__ int qidx(const Geom& g, int i, int l, int j, int k) {
-  return i + g.nx * (l + 5 * (j + g.ny * k));
+// state index of (i,l,j,k), 0-based: 64-bit, since the state holds five
+// values per cell and a grid of up to 2^31 cells fits one GPU's HBM (the
+// bracket, 5 * (j + ny * k) + l, stays below 5 * ny * nz)
+__device__ __forceinline__ long long qidx(const Geom& g, int i, int l, int j, int k) {
+  return i + (long long)g.nx * (l + 5 * (j + g.ny * k));
 }
 __device__ __forceinline__ int cidx(const Geom& g, int i, int j, int k) { return i + g.nx * (j + g.ny * k); }
 
@@ -112,7 +116,7 @@ __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ QJ, Pri
   const int j = blockIdx.y * 4 + threadIdx.y;
   const int k = blockIdx.z;
   if (i >= g.nx || j >= g.ny) return;
-  const int q = qidx(g, i, 0, j, k);
+  const long long q = qidx(g, i, 0, j, k);
   const int st = g.nx;
   const PrimVals r = prim_of(QJ[q], QJ[q + st], QJ[q + 2 * st], QJ[q + 3 * st], QJ[q + 4 * st], g.jac, c);
   store_prim<S>(P, cidx(g, i, j, k), r);
@@ -819,15 +823,25 @@ constexpr int LINE_X = OX_LINE_X;
 constexpr int LINE_Y = 256 / LINE_X;
 static_assert(LINE_X % 64 == 0 && LINE_X * LINE_Y == 256, "line-sweep workgroup of whole wavefronts");
 
+// Longest segment of the y (dir 1) or z (dir 2) sweep whose state offsets
+// from a thread's column pointer (the cell before the segment) stay below
+// 2^31: a thread reaches lseg + 1 cells along dir past it, plus the five
+// variables of a cell, so (lseg + 2) cells of 5 nx (y) or 5 nx ny (z) values.
+// At least 1 for any grid ox_create accepts (15 nx ny < 2^31).
+inline int lseg_limit(const Geom& g, int dir) {
+  const long long step = dir == 1 ? 5LL * g.nx : 5LL * g.nx * g.ny;
+  const long long n = 2147483647LL / step - 2;
+  return (int)std::max(1LL, std::min(n, 4096LL));
+}
+inline int clamp_lseg(const Geom& g, int dir, int lseg) { return std::min(lseg, lseg_limit(g, dir)); }
+
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO, int DIR, int EP>
 __device__ __forceinline__ void sweep_line_tile(const SweepArgs<R, S>& a, int bx, int by, int bz) {
   const Prim<S>& P = a.P;
   const Prim<S>& Pn = a.Pn;
-  S* __restrict__ Racc = a.Racc;
   const Geom& g = a.g;
   const PhysC<R>& c = a.c;
   const PhysC<double>& c64 = a.c64;
-  const StageArgs& sa = a.sa;
   const int tvd = a.tvd, variant = a.variant;
   const double cd = a.cd;
   using St = Sten<ORDER, VISC, VO>;
@@ -841,7 +855,20 @@ __device__ __forceinline__ void sweep_line_tile(const SweepArgs<R, S>& a, int bx
   const int s_end = min(s0 + a.lseg, g.h + nd);
   const int stride = DIR == 1 ? g.nx : g.nx * g.ny;
   const int mbase = DIR == 1 ? cidx(g, i, 0, t) : cidx(g, i, t, 0);
-  const int qbase = DIR == 1 ? qidx(g, i, 0, 0, t) : qidx(g, i, 0, t, 0);
+  // State, residual and RK operands: each thread's pointers to its column
+  // at the cell before the segment (64-bit: the state of a large grid spans
+  // more than 2^31 values), and offsets along DIR that are the same for the
+  // whole workgroup (scalar registers, 32-bit: below 2^31 by the
+  // segment-length limit, lseg_limit).
+  const long long qcol = DIR == 1 ? qidx(g, i, 0, s0 - 1, t) : qidx(g, i, 0, t, s0 - 1);
+  S* __restrict__ Racc = a.Racc + qcol;
+  StageArgs sa = a.sa;
+  if constexpr (DIR == 2) {
+    // only the operands this epilogue touches (the others may be null)
+    if constexpr (EP == EP_RK3_S1 || EP == EP_RK3_S23 || EP == EP_RK4_S) sa.Qin += qcol;
+    sa.Qout += qcol;
+    if constexpr (EP == EP_RK4_S || EP == EP_RK4_S4) sa.Rs += qcol;
+  }
   const int qstride = DIR == 1 ? 5 * g.nx : 5 * g.nx * g.ny;
   const R inv_d = DIR == 1 ? R(g.inv_dy) : R(g.inv_dz);
   // the sensor is stored for interior cells only; a ghost cell's sensor is
@@ -889,8 +916,8 @@ __device__ __forceinline__ void sweep_line_tile(const SweepArgs<R, S>& a, int bx
     }
     // residual (and state) of cell f+1, consumed one iteration later
     CellState<DIR, EP> ncs;
-    fetch_state<DIR, EP>(Racc, sa, qbase + (f + 1) * qstride, g.nx, ncs);
-    const int q = qbase + f * qstride;  // cell f
+    const int q = (f - s0 + 1) * qstride;  // cell f, from the column pointers
+    fetch_state<DIR, EP>(Racc, sa, q + qstride, g.nx, ncs);
     const double *Ra = cs.R, *Qa = cs.A, *Qb = cs.B, *Qc = cs.C;
     R F[5];
     face_flux<R, DIR, ORDER, SCHEME, VISC, VO>(c, tvd, variant, inv_d, win, F);

@@ -39,3 +39,35 @@ def test_config_struct_layout():
     # 17 int32 then 9 doubles; doubles 8-aligned
     assert ctypes.sizeof(_abi.OxConfig) == 17 * 4 + 4 + 9 * 8
     assert ctypes.sizeof(_abi.OxDiag) == 6 * 8
+
+
+def test_grid_limits_refused_without_gpu():
+    """ox_create takes grids up to 2^31 cells (64-bit state indices) and xy
+    planes up to 2^31 / 15 cells (the z sweep's 32-bit offsets from a column
+    pointer span at least three planes); beyond them it refuses before any
+    HIP call."""
+    from ouxsbli_amd.cases import CaseSpec
+    L = _native.load()
+    h = ctypes.c_void_p()
+    too_many = CaseSpec(nx=1300, ny=1300, nz=1300, order=6, visc="Euler").to_ox()
+    assert L.ox_create(ctypes.byref(too_many), ctypes.byref(h)) == _abi.OX_EINVAL
+    assert b"2^31 cells" in L.ox_last_error()
+    wide = CaseSpec(nx=12000, ny=12000, nz=6, order=2, visc="Euler").to_ox()
+    assert L.ox_create(ctypes.byref(wide), ctypes.byref(h)) == _abi.OX_EINVAL
+    assert b"xy plane" in L.ox_last_error()
+
+
+def test_footprint_of_a_grid_beyond_32_bit_state_indices():
+    """760^3 with ghosts holds 5 * 760^3 > 2^31 state values; its FP64 NS RK3
+    footprint (two states, the residual, two sets of six primitive arrays)
+    fits one MI355X's 288 GB."""
+    from ouxsbli_amd.cases import CaseSpec
+    L = _native.load()
+    spec = CaseSpec(nx=760, ny=760, nz=760, order=6, visc="NS", rk=3)
+    b = ctypes.c_uint64()
+    assert L.ox_footprint(ctypes.byref(spec.to_ox()), ctypes.byref(b)) == _abi.OX_OK
+    cells = 760**3
+    assert 5 * cells > 2**31
+    assert b.value >= cells * (2 * 40 + 40 + 2 * 6 * 8)
+    assert b.value < cells * (2 * 40 + 40 + 2 * 6 * 8) + (1 << 20)
+    assert b.value < 288e9

@@ -231,8 +231,10 @@ int validate(const ox_config& c) {
   if ((c.bc_x || c.bc_y || c.bc_z) && (c.visc == OX_VISC_LES || c.scheme == OX_SCHEME_ROE))
     // an LES case's ghost fill of nu_t and qc2 next to walls is its own
     // set_bc_mut (3D_solver/TBL/set.f90:142-185), not built; the reference's
-    // non-periodic Roe kernels are first order on every face
-    // (3D_solver/src/calc_roe_kernel.f90.fypp:43-92)
+    // non-periodic Roe kernels compare the half-order io (0..2) with 4 and 6,
+    // so they are first order on every face, and take an id_accuracy argument
+    // their call site does not pass (3D_solver/src/calc_roe_kernel.f90.fypp:
+    // 31-92, calc_flux_base.f90.fypp:60): that path does not build
     return ox_fail(OX_EUNSUPPORTED, "non-periodic boundaries are built for Euler and NS with KEEP, SLAU and Hybrid");
   if (c.visc_stencil != OX_STENCIL_CENTRAL && c.visc_stencil != OX_STENCIL_ME4BASE)
     return ox_fail(OX_EINVAL, "unknown viscous stencil");

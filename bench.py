@@ -111,7 +111,7 @@ def host_cpu():
 
 
 def _oracle_rate(spec, nsteps, repeats):
-    """BASELINE.md protocol: 1 warm-up step, then `repeats` timings of
+    """Timing protocol: 1 warm-up step, then `repeats` timings of
     `nsteps` steps with a steady clock; the median rate (Mcell-updates/s per
     RK stage).  Setup and I/O excluded."""
     from ouxsbli_amd import cases

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define OX_ABI_VERSION 1
+#define OX_ABI_VERSION 2
 
 /* Error codes. */
 #define OX_OK            0
@@ -401,21 +401,34 @@ int ox_batch_admit2d(ox_batch* b, const ox2d_config* cfg, const double* Q, int32
                                 at x = 2 pi (interior index) / n, w_0 = 1, w_kx = 2 otherwise (the
                                 inverse real transform of the half spectrum), then scaled so that
                                 sqrt(<|u|^2>) = V0; uniform density rho0 and pressure p0            */
+#define OX_IC_ETGV 4         /* ETGV set_init (3D_solver/ETGV/set.f90:27-40): rho = rho0,
+                                rho u = rho0 M0 sin x cos y cos z, rho v = -rho0 M0 cos x sin y cos z,
+                                w = 0, p = 1/gamma + rho0 M0^2 (cos 2x + cos 2y)(cos 2z + 2)/16,
+                                with M0 = V0, at the cell centres of the [0, 2 pi) box               */
+#define OX_IC_KHI 5          /* KHI set_init (3D_solver/KHI/set.f90:31-47): stream 1 (V0, rho0) where
+                                y > 0.75 Lx or y < 0.25 Lx (Lx, as the reference writes it), stream 2
+                                (u2, rho2) between, pressure p0, perturbations
+                                v = amp V0 sin(2 pi x / Lx), w = amp V0 sin(2 pi z / Lz)             */
 
 typedef struct ox_ic {
   int32_t kind;     /* OX_IC_*                                                                  */
   int32_t kmax;     /* spectral: K of the mode box (K < n/2); 0 for the other kinds             */
   uint64_t seed;    /* shear layer: noise seed                                                  */
   double L0;        /* Taylor-Green: length scale of sin(x/L0), cos(2x/L0), ...                 */
-  double V0;        /* Taylor-Green: V0; shear layer: U, the velocity difference across a layer */
-  double rho0;      /* Taylor-Green: RHO0 of the pressure amplitude; shear layer: density       */
-  double p0;        /* Taylor-Green: mean pressure; shear layer: pressure                       */
+  double V0;        /* Taylor-Green: V0; shear layer: U, the velocity difference across a layer;
+                       ETGV: M0; KHI: u1                                                        */
+  double rho0;      /* Taylor-Green: RHO0 of the pressure amplitude; shear layer: density;
+                       ETGV: RHO0; KHI: rho1                                                    */
+  double p0;        /* Taylor-Green: mean pressure; shear layer, KHI: pressure                  */
   double T0;        /* Taylor-Green: temperature, rho = p / (R T0)                              */
   double delta;     /* shear layer: thickness d of tanh((y - Ly/4) / d)                         */
   double noise;     /* shear layer: amplitude of the uniform [-1, 1) velocity noise             */
   const double* modes; /* spectral: host array H[3][K+1][2K+1][2K+1] of complex coefficients
                           (re, im interleaved; component, kx, ky + K, kz + K); copied during the
                           call                                                                     */
+  double u2;        /* KHI: velocity of stream 2 (V0, rho0 are stream 1's; p0 the pressure)     */
+  double rho2;      /* KHI: density of stream 2                                                 */
+  double amp;       /* KHI: perturbation amplitude                                              */
 } ox_ic;
 
 /* Builds the initial state of `ic` in the handle (interior at the cell

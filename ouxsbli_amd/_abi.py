@@ -4,7 +4,7 @@ Only layout and constants live here; the loader is ``_native.py``.
 """
 import ctypes
 
-OX_ABI_VERSION = 1
+OX_ABI_VERSION = 2
 
 OX_OK = 0
 OX_EINVAL = -1
@@ -90,6 +90,8 @@ class Ox2dConfig(ctypes.Structure):
 OX_IC_TAYLOR_GREEN = 1
 OX_IC_SHEAR_LAYER = 2
 OX_IC_SPECTRAL = 3
+OX_IC_ETGV = 4
+OX_IC_KHI = 5
 
 
 class OxIc(ctypes.Structure):
@@ -107,6 +109,9 @@ class OxIc(ctypes.Structure):
         ("delta", ctypes.c_double),
         ("noise", ctypes.c_double),
         ("modes", ctypes.POINTER(ctypes.c_double)),
+        ("u2", ctypes.c_double),
+        ("rho2", ctypes.c_double),
+        ("amp", ctypes.c_double),
     ]
 
 

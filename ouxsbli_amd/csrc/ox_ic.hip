@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void k_zero(double* __restrict__ a, long long 
 
 struct IcArgs {
   double dx1, dy1, dz1;  // node spacings L / (n - order)
-  double Ly, gm1, R, J;
+  double Lx, Ly, Lz, gamma, gm1, R, J;
   ox_ic ic;
 };
 
@@ -78,6 +78,31 @@ __global__ __launch_bounds__(256) void k_ic(double* __restrict__ QJ, Geom g, IcA
     q[2] = -rho * V0 * cos(X / L0) * sin(Y / L0) * cos(Z / L0);
     q[3] = 0.0;
     q[4] = p / a.gm1 + 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0];
+  } else if (ic.kind == OX_IC_ETGV) {
+    // ETGV/set.f90:27-40 (the reference's expressions and operation order)
+    const double X = centre(i, g.h, a.dx1), Y = centre(j, g.h, a.dy1), Z = centre(k, g.h, a.dz1);
+    const double rho0 = ic.rho0, M0 = ic.V0;
+    q[0] = rho0;
+    q[1] = rho0 * M0 * sin(X) * cos(Y) * cos(Z);
+    q[2] = -(rho0 * M0 * cos(X) * sin(Y) * cos(Z));
+    q[3] = 0.0;
+    q[4] = (1.0 / a.gamma + 0.0625 * rho0 * (M0 * M0) * (cos(2.0 * X) + cos(2.0 * Y)) * (cos(2.0 * Z) + 2.0)) /
+               (a.gamma - 1.0) +
+           0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / q[0];
+  } else if (ic.kind == OX_IC_KHI) {
+    // KHI/set.f90:31-47: the stream test uses Lx, as the reference writes it
+    const double X = centre(i, g.h, a.dx1), Y = centre(j, g.h, a.dy1), Z = centre(k, g.h, a.dz1);
+    const double pi = 3.141592653589793;  // acos(-1.d0)
+    const double u1 = ic.V0, rho1 = ic.rho0;
+    const double w = ic.amp * u1 * sin(2.0 * pi * Z / a.Lz);
+    const double v = ic.amp * u1 * sin(2.0 * pi * X / a.Lx);
+    const bool outer = Y > 0.75 * a.Lx || Y < 0.25 * a.Lx;
+    const double rho = outer ? rho1 : ic.rho2, u = outer ? u1 : ic.u2;
+    q[0] = rho;
+    q[1] = rho * u;
+    q[2] = rho * v;
+    q[3] = rho * w;
+    q[4] = ic.p0 / (a.gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
   } else {  // OX_IC_SHEAR_LAYER
     const double y = centre(j, g.h, a.dy1);
     const double U = ic.V0, d = ic.delta;
@@ -359,7 +384,8 @@ extern "C" {
 int ox_init_state(ox_solver* s, const ox_ic* ic) {
   if (!s || !ic) return ox_fail(OX_EINVAL, "null argument");
   if (ic->kind == OX_IC_SPECTRAL) return init_spectral(s, ic);
-  if (ic->kind != OX_IC_TAYLOR_GREEN && ic->kind != OX_IC_SHEAR_LAYER)
+  if (ic->kind != OX_IC_TAYLOR_GREEN && ic->kind != OX_IC_SHEAR_LAYER && ic->kind != OX_IC_ETGV &&
+      ic->kind != OX_IC_KHI)
     return ox_fail(OX_EINVAL, "unknown initial condition kind");
   if (ic->kind == OX_IC_TAYLOR_GREEN && (ic->L0 <= 0.0 || ic->T0 <= 0.0))
     return ox_fail(OX_EINVAL, "Taylor-Green needs L0 > 0 and T0 > 0");
@@ -372,7 +398,10 @@ int ox_init_state(ox_solver* s, const ox_ic* ic) {
   a.dx1 = c.Lx / (double)(c.nx - c.order);
   a.dy1 = c.Ly / (double)(c.ny - c.order);
   a.dz1 = c.Lz / (double)(c.nz - c.order);
+  a.Lx = c.Lx;
   a.Ly = c.Ly;
+  a.Lz = c.Lz;
+  a.gamma = c.gamma;
   a.gm1 = c.gamma - 1.0;
   a.R = c.R;
   a.J = g.jac;

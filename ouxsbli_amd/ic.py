@@ -201,15 +201,26 @@ CASE_BOUNDARIES = {"STZ": ("slip", "slip", "extrapolate")}
 
 def device_ic(case_name: str, spec, g: dict):
     """The ``ox_ic`` that builds the case's set_init on the device, or None
-    where only the host form exists.  NSTGV's set_init (3D_solver/NSTGV/
-    set.f90:16-47) is the library's OX_IC_TAYLOR_GREEN with the case file's
-    l0, v0, rho0, p0 and t; an explicit ic= of the Case, and every other
-    case, stay on the host."""
-    if g.get("ic_type") is not None or case_name != "NSTGV":
+    where only the host form exists:
+      NSTGV (3D_solver/NSTGV/set.f90:16-47): OX_IC_TAYLOR_GREEN with the case
+        file's l0, v0, rho0, p0 and t;
+      ETGV (3D_solver/ETGV/set.f90:27-40): OX_IC_ETGV with rho0 and m0;
+      KHI (3D_solver/KHI/set.f90:31-47): OX_IC_KHI with u1, rho1, u2, rho2,
+        p and amp.
+    An explicit ic= of the Case, and the other cases, stay on the host."""
+    if g.get("ic_type") is not None:
         return None
-    spec.ic = "tgv"
-    spec.L0, spec.V0, spec.RHO0, spec.p0, spec.T0 = g["l0"], g["v0"], g["rho0"], g["p0"], g["t"]
-    return spec.device_ic()
+    from . import _abi
+    if case_name == "NSTGV":
+        spec.ic = "tgv"
+        spec.L0, spec.V0, spec.RHO0, spec.p0, spec.T0 = g["l0"], g["v0"], g["rho0"], g["p0"], g["t"]
+        return spec.device_ic()
+    if case_name == "ETGV":
+        return _abi.OxIc(kind=_abi.OX_IC_ETGV, rho0=float(g["rho0"]), V0=float(g["m0"]))
+    if case_name == "KHI":
+        return _abi.OxIc(kind=_abi.OX_IC_KHI, V0=float(g["u1"]), rho0=float(g["rho1"]), u2=float(g["u2"]),
+                         rho2=float(g["rho2"]), p0=float(g["p"]), amp=float(g["amp"]))
+    return None
 
 
 def initial_state(case_name: str, spec, g: dict) -> np.ndarray:

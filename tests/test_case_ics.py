@@ -92,3 +92,25 @@ def test_khi_state_matches_set_init(order):
     inner = (y[h:-h] > 0.25 * spec.Lx) & (y[h:-h] < 0.75 * spec.Lx)
     rho_col = Q[h + 1, h:-h, 0, h + 1]
     np.testing.assert_array_equal(rho_col, np.where(inner, 2.0, 1.0))
+
+
+def test_device_ic_routing():
+    """NSTGV, ETGV and KHI build their set_init on the device (ic.device_ic);
+    an explicit ic= and the other cases stay on the host."""
+    from ouxsbli_amd import _abi
+    spec = _spec((12, 10, 14), 4, (2 * math.pi, 2 * math.pi, 2 * math.pi))
+    d = ic.device_ic("ETGV", spec, {"rho0": 1.2, "m0": 0.1})
+    assert (d.kind, d.rho0, d.V0) == (_abi.OX_IC_ETGV, 1.2, 0.1)
+    g = {"u1": 0.5, "rho1": 1.0, "u2": -0.5, "rho2": 2.0, "p": 2.5, "amp": 0.01}
+    d = ic.device_ic("KHI", spec, g)
+    assert (d.kind, d.V0, d.rho0, d.u2, d.rho2, d.p0, d.amp) == (_abi.OX_IC_KHI, 0.5, 1.0, -0.5, 2.0, 2.5, 0.01)
+    assert ic.device_ic("KHI", spec, {**g, "ic_type": 1}) is None
+    assert ic.device_ic("STZ", spec, {}) is None
+
+
+def test_ox_ic_struct_layout():
+    """ox_ic: two int32, a uint64, seven doubles, a pointer, three doubles."""
+    import ctypes
+    from ouxsbli_amd import _abi
+    assert ctypes.sizeof(_abi.OxIc) == 4 + 4 + 8 + 7 * 8 + 8 + 3 * 8
+    assert _abi.OxIc.u2.offset == 4 + 4 + 8 + 7 * 8 + 8

@@ -122,3 +122,24 @@ def test_etgv_case_like_the_reference_test(tmp_path):
     assert ent.shape == (2, 2) and ke.shape == (2, 3)
     assert abs(ent[-1, -1]) < 3.0e-3, ent
     assert 0.75 < ke[-1, -1] < 1.0 + 0.01, ke
+
+
+def test_khi_case_from_device_state(tmp_path):
+    """Case("3D_solver/KHI") builds the KHI set_init on the device
+    (OX_IC_KHI); its run agrees with the same run from the host form of the
+    state to 1e-12 of each variable's scale (the device's sin rounds within an
+    ulp of the host's)."""
+    from ouxsbli_amd import ic
+    from ouxsbli_amd.solver import Solver
+    c = Case("3D_solver/KHI", str(tmp_path / "khi"), nx=24, ny=24, nz=24, np=1, nt=4)
+    c.build()
+    assert ic.device_ic("KHI", c.spec, c.globals) is not None
+    c.run()
+    got = io_vtr.read_restart(tmp_path / "khi" / "recal" / "Q00001.dat", c.spec.state_shape())
+    with Solver(c.spec) as s:
+        s.set_state(ic.khi_state(c.spec, c.globals))
+        s.step(4)
+        ref = s.get_state()
+    for l in range(5):
+        scale = np.abs(ref[:, :, l, :]).max()
+        assert np.abs(got[:, :, l, :] - ref[:, :, l, :]).max() <= 1e-12 * scale, l

@@ -9,11 +9,13 @@ the same state (RK4 accumulator zero, primitives rebuilt); and a case
 admitted to a batch from its parameters ends bit-identical to the same case
 advanced alone from the same device-built state."""
 import dataclasses
+import math
 
 import numpy as np
 import pytest
 
-from ouxsbli_amd import cases, cases2d as c2
+from ouxsbli_amd import cases, cases2d as c2, ic
+from ouxsbli_amd.cases import CaseSpec
 from ouxsbli_amd.batch import Batch
 from ouxsbli_amd.solver import Solver
 from ouxsbli_amd.solver2d import Solver2D
@@ -63,6 +65,43 @@ def test_shear_layer_matches_host(n, order, seed):
     ref = _uploaded(spec, cases.khi_state(spec))
     # rho, rho v, rho w hold no transcendental function: bit for bit
     for l in (0, 2, 3):
+        assert np.array_equal(got[:, :, l, :], ref[:, :, l, :]), f"variable {l}"
+    _close(got, ref, 1e-14)
+
+
+def _ref_case_spec(n, order, L):
+    (nx, ny, nz), (Lx, Ly, Lz) = n, L
+    return CaseSpec(nx=nx + order, ny=ny + order, nz=nz + order, order=order, scheme="KEEP", visc="Euler",
+                    tvd="none", rk=3, Lx=Lx, Ly=Ly, Lz=Lz)
+
+
+def _device_case_state(spec, case, g):
+    with Solver(spec) as s:
+        s.init_state(ic.device_ic(case, spec, g))
+        return s.get_state()
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_etgv_matches_host(order):
+    """ETGV's set_init on the device (OX_IC_ETGV) against its host form."""
+    spec = _ref_case_spec((24, 20, 28), order, (2 * math.pi,) * 3)
+    g = {"rho0": 1.0, "m0": 0.1}
+    got = _device_case_state(spec, "ETGV", g)
+    ref = _uploaded(spec, ic.etgv_state(spec, g))
+    for l in (0, 3):  # rho, rho w: no transcendental function
+        assert np.array_equal(got[:, :, l, :], ref[:, :, l, :]), f"variable {l}"
+    _close(got, ref, 1e-14)
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_khi_matches_host(order):
+    """KHI's set_init on the device (OX_IC_KHI) against its host form, with
+    Ly = 1.6 Lx: the stream test reads Lx, as the reference writes it."""
+    spec = _ref_case_spec((32, 52, 24), order, (1.0, 1.6, 0.8))
+    g = {"u1": 0.5, "rho1": 1.0, "u2": -0.5, "rho2": 2.0, "p": 2.5, "amp": 0.01}
+    got = _device_case_state(spec, "KHI", g)
+    ref = _uploaded(spec, ic.khi_state(spec, g))
+    for l in (0, 1):  # rho, rho u: the streams' constants
         assert np.array_equal(got[:, :, l, :], ref[:, :, l, :]), f"variable {l}"
     _close(got, ref, 1e-14)
 

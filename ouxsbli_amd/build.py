@@ -6,10 +6,11 @@ The FP64 units are compiled with -ffp-contract=off (every multiply and add
 rounded separately, in the evaluation order of the reference expressions);
 the FP32-bulk units contract a*b+c within one source expression
 (-ffp-contract=on), a rule that does not depend on the surrounding code, so
-the single-case and the batched launches of a case round identically, and
-take FP32 division and square root in their fast hardware forms (about 1 ulp
-instead of correctly rounded).  Packed FP32 instructions are disabled for all
-device code.
+the single-case and the batched launches of a case round identically, take
+FP32 division and square root in their fast hardware forms (about 1 ulp
+instead of correctly rounded) and flush FP32 denormals to zero (FP64
+arithmetic keeps its denormals).  Packed FP32 instructions are disabled for
+all device code.
 """
 from __future__ import annotations
 
@@ -33,17 +34,21 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unknown-pragmas",
     f"-I{ROOT / 'include'}",
 ]
+# FP32-bulk units: contraction within an expression, fast FP32 division and
+# square root, FP32 denormals flushed to zero (which lets the fast division be
+# a reciprocal and a multiply, without the range scaling denormals need)
+FP32_BULK = ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"]
 UNITS = {
     "ox_capi.hip": ["-ffp-contract=off"],
     "ox_batch.hip": ["-ffp-contract=off"],
     "ox_fp64.hip": ["-ffp-contract=off"],
     "ox_fp64_batch.hip": ["-ffp-contract=off"],
-    "ox_fp32.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
-    "ox_fp32_batch.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "ox_fp32.hip": FP32_BULK,
+    "ox_fp32_batch.hip": FP32_BULK,
     "ox2d_capi.hip": ["-ffp-contract=off"],
     "ox_ic.hip": ["-ffp-contract=off"],
     "ox2d_fp64.hip": ["-ffp-contract=off"],
-    "ox2d_fp32.hip": ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "ox2d_fp32.hip": FP32_BULK,
 }
 
 

@@ -2,7 +2,8 @@
 // ox2d_init_state (2-D oblique shock).
 //
 // The reference samples the initial field at the interior cell centres on the
-// host (3D_solver/NSTGV/set.f90:16-47, 2D_solver/OS/set.f90:28-46), fills the
+// host (3D_solver/NSTGV/set.f90:16-47, ETGV/set.f90:27-40, KHI/set.f90:31-47,
+// 2D_solver/OS/set.f90:28-46), fills the
 // periodic ghosts (3D_solver/src/set_bc_common.f90:11-97) and divides by the
 // Jacobian in pre_calc (3D_solver/src/preprocess.f90.fypp:65-71).  Here one
 // kernel writes QJ = Q / J at every interior cell and k_ghost_fill copies the
@@ -12,8 +13,9 @@
 // dx1 * (i - order/2) with dx1 = L / (n - order), centre = the midpoint of two
 // nodes, set_grid_cyclic{2,4,6}_3D in src/set_coordinate.f90:120-339), and the
 // field expressions keep the operation order of the host restatement in
-// ouxsbli_amd/cases.py, so the two differ only where the device's sin, cos and
-// tanh round differently from the host's (a few ulp).
+// ouxsbli_amd/cases.py (ETGV and KHI: that of the reference's set_init), so
+// the two differ only where the device's sin, cos and tanh round differently
+// from the host's (a few ulp).
 #include <hip/hip_runtime.h>
 
 #include <cmath>

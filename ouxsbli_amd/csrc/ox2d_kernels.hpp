@@ -67,9 +67,9 @@ template <class R> struct Args2 {
 
 constexpr int TX = 64;
 // Rows per tile.  11 FP32 rows: 715 x faces, 768 y faces and 704 cells
-// keep the 256 threads busy in three passes each, and 34 KiB of LDS (106
-// VGPRs) lets four workgroups share a CU.  4 FP64 rows: 35 KiB, three per
-// CU (130 VGPRs).
+// keep the 256 threads busy in three passes each, and 34 KiB of LDS (107
+// VGPRs) lets four workgroups share a CU.  4 FP64 rows: 35 KiB of LDS and
+// 124 VGPRs, also four per CU.
 #ifndef OX2_ROWS_FP64
 #define OX2_ROWS_FP64 4
 #endif

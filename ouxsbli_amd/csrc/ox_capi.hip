@@ -373,6 +373,7 @@ int ox_create(const ox_config* cfg, ox_solver** out) {
     for (int m = 0; m < ox::kPrimArrays; ++m) C.prim[set][m] = nullptr;
   C.cur = 0;
   C.prim_for = nullptr;
+  C.prim_images[0] = C.prim_images[1] = false;
   C.stream = nullptr;
   C.lseg[0] = ox::clamp_lseg(C.g, 1, ox::LSEG_DEFAULT);
   C.lseg[1] = ox::clamp_lseg(C.g, 2, ox::LSEG_DEFAULT);

@@ -23,6 +23,10 @@ struct Core {
   void* prim[2][8];
   int cur;
   const double* prim_for;
+  // set s was last written by a z-sweep epilogue, which stores at every ghost
+  // cell its periodic image's primitives (k_prim instead takes the ghost
+  // layers of the state as they are)
+  bool prim_images[2];
   // residual accumulator between the sweeps, state layout, element type S
   // (FP64 in the plain form; FP32 in the FP32-bulk form, where it carries
   // FP32 flux differences and the RK combination itself stays FP64)
@@ -60,6 +64,7 @@ inline void finish_stage(Core& C, const StageArgs& sa) {
   if (sa.ep != EP_RHS) {
     C.cur = 1 - C.cur;
     C.prim_for = sa.Qout;
+    C.prim_images[C.cur] = true;
   }
 }
 

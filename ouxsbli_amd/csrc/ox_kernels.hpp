@@ -759,6 +759,156 @@ __global__ __launch_bounds__(256) void k_sweep_x(SweepArgs<R, S> a) {
   sweep_x_tile<R, S, ORDER, SCHEME, VISC, VO>(a, bx, by, bz, tile);
 }
 
+// ---------------------------------------------------------------------------
+// x sweep over whole rows of a periodic grid whose interior width ix is 64,
+// 128 or 256.  A row tile above spends SW - 1 of its 64 lanes on halo cells,
+// so a row of 256 cells at SW = 6 takes five waves, the fifth with 24 cells.
+// Here a workgroup of 256 lanes holds 256 / ix whole rows, one lane per cell
+// and one face per lane (its cell's left face), and the halo positions of the
+// LDS row hold the periodic images of the row's other end.  The face left of
+// the first cell is then the face right of the last: its window cells are the
+// periodic images of that face's window cells.  The row tile computes the
+// same face from the ghost cells, which hold exact copies of those images
+// once a z-sweep epilogue has written the primitive set, so both tiles give
+// the same bits.  Used for such primitive sets only (k_prim takes the ghost
+// layers of an uploaded state as they are), on fully periodic grids, without
+// LES (its ghost SGS fields come from k_les_bc), and not for ORDER 2 with the
+// sensor, whose ghost cells copy the adjacent interior sensor
+// (calc_hybrid.f90:54-71) instead of the image's.
+// ---------------------------------------------------------------------------
+template <int ORDER, int SCHEME, int VISC>
+constexpr bool kXRowSweep = VISC != 2 && !(kUsesSensor<SCHEME, ORDER> && ORDER == 2);
+__host__ __device__ constexpr bool xrow_width(int ix) { return ix == 64 || ix == 128 || ix == 256; }
+
+template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
+__global__ __launch_bounds__(256) void k_sweep_xrow(SweepArgs<R, S> a) {
+  using St = Sten<ORDER, VISC, VO>;
+  using CD = CellData<R, 0, ORDER, SCHEME, VISC, VO>;
+  using RC = RawCell<S, VISC, VO>;
+  constexpr bool SENS = kUsesSensor<SCHEME, ORDER>;
+  constexpr int NF = xsweep_nf<VISC>();
+  constexpr int HW = St::HW, SW = St::SW;
+  static_assert(kXRowSweep<ORDER, SCHEME, VISC>, "row sweep for image-ghost schemes only");
+  // rows of L + SW - 1 positions: 4 * (63 + SW) covers L = 64 (four rows) to 256 (one)
+  __shared__ R tile[NF][4 * (63 + SW)];
+  __shared__ double edge[4][5];  // each wave's lane-0 face flux
+  const Prim<S>& P = a.P;
+  const Geom& g = a.g;
+  const PhysC<R>& c = a.c;
+  const int L = g.ix, rw = 256 / L;  // row length, rows per workgroup
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int r = tid / L, p = tid - r * L;  // row in the workgroup, cell in the row
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int j = g.h + (t / g.iz) * rw + r, k = g.h + t % g.iz;
+  const bool rowok = j < g.h + g.iy;
+  const int i = g.h + p;
+  const int rb = r * (L + SW - 1);  // the row's first position, cell h - HW
+  const int m = p + HW;             // this lane's position
+  const int q = cidx(g, i, j, k);
+  DucrosPart<R> dp{};
+  if (rowok) {
+    RC rc;
+    fetch_cell<S, 0, ORDER, SCHEME, VISC, VO, false>(P, g, q, 0, rc);
+    if constexpr (SENS) {
+      const int sy = g.nx, sz = g.nx * g.ny;
+      constexpr int H = vcross<VO>() / 2;
+      R uym, uyp, vym, vyp, uzm, uzp, wzm, wzp;
+      if constexpr (VISC) {
+        uym = R(rc.x[0][H - 1]); uyp = R(rc.x[0][H]);
+        vym = R(rc.x[1][H - 1]); vyp = R(rc.x[1][H]);
+        uzm = R(rc.x[2][H - 1]); uzp = R(rc.x[2][H]);
+        wzm = R(rc.x[3][H - 1]); wzp = R(rc.x[3][H]);
+      } else {
+        uym = R(G(P.u)[q - sy]); uyp = R(G(P.u)[q + sy]);
+        vym = R(G(P.v)[q - sy]); vyp = R(G(P.v)[q + sy]);
+        uzm = R(G(P.u)[q - sz]); uzp = R(G(P.u)[q + sz]);
+        wzm = R(G(P.w)[q - sz]); wzp = R(G(P.w)[q + sz]);
+      }
+      dp = ducros_yz<R>(g, uym, uyp, vym, vyp, R(G(P.w)[q - sy]), R(G(P.w)[q + sy]), uzm, uzp, R(G(P.v)[q - sz]),
+                        R(G(P.v)[q + sz]), wzm, wzp);
+    }
+    CD d;
+    finish_cell<R, S, 0, ORDER, SCHEME, VISC, VO, false>(g, c, rc, d);
+    auto put = [&](int at) {
+      at += rb;
+      tile[0][at] = d.rho; tile[1][at] = d.u; tile[2][at] = d.v; tile[3][at] = d.w; tile[4][at] = d.p;
+      if constexpr (kUsesT<SCHEME> || VISC) tile[5][at] = d.T;
+      if constexpr (VISC) {
+        tile[6][at] = d.mu;
+        tile[8][at] = d.d1; tile[9][at] = d.d2; tile[10][at] = d.d3; tile[11][at] = d.d4;
+      }
+    };
+    put(m);
+    if (p >= L - HW) put(m - L);      // left halo: images of the row's last HW cells
+    if (p < SW - HW - 1) put(m + L);  // right halo: images of its first SW - HW - 1
+  }
+  __syncthreads();
+  if constexpr (SENS) {
+    if (rowok) {
+      const int at = rb + m;
+      const R fd = ducros_x<R>(g, dp, tile[1][at - 1], tile[1][at + 1], tile[2][at - 1], tile[2][at + 1],
+                               tile[3][at - 1], tile[3][at + 1]);
+      tile[7][at] = fd;
+      if (p == L - 1) tile[7][rb + HW - 1] = fd;  // the image left of the row
+      G(P.s)[q] = (S)fd;
+    }
+    __syncthreads();
+  }
+  R F[5] = {R(0), R(0), R(0), R(0), R(0)};
+  if (rowok) {
+    const int w0 = rb + p;  // window cells h + p - HW .. h + p + HW - 1
+    {
+      CD win[SW];
+#pragma unroll
+      for (int n = 0; n < SW; ++n) {
+        win[n].rho = tile[0][w0 + n]; win[n].u = tile[1][w0 + n]; win[n].v = tile[2][w0 + n];
+        win[n].w = tile[3][w0 + n]; win[n].p = tile[4][w0 + n];
+        if constexpr (kUsesT<SCHEME>) win[n].T = tile[5][w0 + n];
+        if constexpr (SENS) {
+          if (n == HW - 1 || n == HW) win[n].s = tile[7][w0 + n];
+        }
+      }
+      face_conv<R, 0, ORDER, SCHEME, VISC, VO>(c, a.tvd, a.variant, win, F);
+    }
+    if constexpr (VISC) {
+      OX_SPLIT_FENCE();
+      CD win[SW];
+#pragma unroll
+      for (int n = 0; n < SW; ++n) {
+        win[n].u = tile[1][w0 + n]; win[n].v = tile[2][w0 + n]; win[n].w = tile[3][w0 + n];
+        win[n].T = tile[5][w0 + n];
+        win[n].mu = tile[6][w0 + n];
+        win[n].d1 = tile[8][w0 + n]; win[n].d2 = tile[9][w0 + n];
+        win[n].d3 = tile[10][w0 + n]; win[n].d4 = tile[11][w0 + n];
+      }
+      face_visc<R, 0, ORDER, SCHEME, VISC, VO>(c, R(g.inv_dx), win, F);
+    }
+  }
+  // the right face is the next cell's left face: the next lane's, across a
+  // wave boundary the next wave's lane 0, and after the row's last cell the
+  // row's first (with L = 64 the row is the wave, and the rotation covers it)
+  double Fr[5];
+#pragma unroll
+  for (int l = 0; l < 5; ++l) Fr[l] = __shfl((double)F[l], (lane + 1) & 63, 64);
+  if (L > 64) {
+    if (lane == 0) {
+#pragma unroll
+      for (int l = 0; l < 5; ++l) edge[tid >> 6][l] = (double)F[l];
+    }
+    __syncthreads();
+    if (lane == 63) {
+      const int wn = (r * L + ((p + 1) & (L - 1))) >> 6;
+#pragma unroll
+      for (int l = 0; l < 5; ++l) Fr[l] = edge[wn][l];
+    }
+  }
+  if (rowok) {
+    const double cx = a.cd;
+#pragma unroll
+    for (int l = 0; l < 5; ++l) G(a.Racc)[qidx(g, i, l, j, k)] = (S)(cx * (-(double)F[l] + Fr[l]));
+  }
+}
+
 // Persistent batched x sweep: block b handles global tiles b, b + gridDim.x, ...
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
 __global__ __launch_bounds__(256) void k_sweep_x_batch(const SweepArgs<R, S>* __restrict__ cases,

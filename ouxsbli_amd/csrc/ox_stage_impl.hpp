@@ -27,6 +27,8 @@ template <class S> Prim<S> prim_set(const Core& C, int set) {
 template <int ORDER, int VISC, int VO> dim3 grid_x(const Geom& g) {
   return dim3(cdiv(g.ix, xtile<ORDER, VISC, VO>()), cdiv(g.iy, XROWS), g.iz);
 }
+// k_sweep_xrow: 256 / ix rows per workgroup, a band of rows in one plane
+inline dim3 grid_xrow(const Geom& g) { return dim3((unsigned)(cdiv(g.iy, 256 / g.ix) * g.iz)); }
 inline dim3 grid_y(const Geom& g, int lseg) {
   return dim3(cdiv(g.ix, LINE_X), cdiv(g.iz, LINE_Y), cdiv(g.iy, lseg));
 }
@@ -46,6 +48,7 @@ void prepare_stage(Core& C, const double* Qsrc) {
     k_prim<S><<<dim3(cdiv(g.nx, 64), cdiv(g.ny, 4), g.nz), blk, 0, C.stream>>>(
         Qsrc, P, g, make_physc<double>(C.hc));
     C.prim_for = Qsrc;
+    C.prim_images[C.cur] = false;
     if (any_bc(g)) launch_bc_sensors<R, S, ORDER, SCHEME>(g, P, C.stream);
     prof_mark(C, "prim");
   }
@@ -89,7 +92,12 @@ void run_stage(Core& C, const double* Qsrc, const StageArgs& sa, double cx, doub
   prepare_stage<R, S, ORDER, SCHEME, VISC, VO>(C, Qsrc);
   SweepArgs<R, S> a[3];
   make_sweep_args<R, S>(C, sa, cx, cy, cz, a);
-  k_sweep_x<R, S, ORDER, SCHEME, VISC, VO><<<grid_x<ORDER, VISC, VO>(g), blk, 0, C.stream>>>(a[0]);
+  bool rows = false;
+  if constexpr (kXRowSweep<ORDER, SCHEME, VISC>) {
+    rows = xrow_width(g.ix) && !any_bc(g) && C.prim_images[C.cur];
+    if (rows) k_sweep_xrow<R, S, ORDER, SCHEME, VISC, VO><<<grid_xrow(g), 256, 0, C.stream>>>(a[0]);
+  }
+  if (!rows) k_sweep_x<R, S, ORDER, SCHEME, VISC, VO><<<grid_x<ORDER, VISC, VO>(g), blk, 0, C.stream>>>(a[0]);
   prof_mark(C, "sweep_x");
   k_sweep_line<R, S, ORDER, SCHEME, VISC, VO, 1, EP_RHS><<<grid_y(g, a[1].lseg), line_block(), 0, C.stream>>>(a[1]);
   prof_mark(C, "sweep_y");

@@ -779,6 +779,12 @@ __global__ __launch_bounds__(256) void k_sweep_x(SweepArgs<R, S> a) {
 template <int ORDER, int SCHEME, int VISC>
 constexpr bool kXRowSweep = VISC != 2 && !(kUsesSensor<SCHEME, ORDER> && ORDER == 2);
 __host__ __device__ constexpr bool xrow_width(int ix) { return ix == 64 || ix == 128 || ix == 256; }
+// Consecutive workgroups take the rows of a band of XROW_BAND rows in one
+// plane, then the next plane: the y neighbours that the cross-derivative and
+// sensor stencils read are shared by workgroups resident together in one L2,
+// and the z neighbours by the bands of the next planes.  (Whole rows one after
+// another through the planes re-read the y neighbours of every row from HBM.)
+constexpr int XROW_BAND = 16;
 
 template <class R, class S, int ORDER, int SCHEME, int VISC, int VO>
 __global__ __launch_bounds__(256) void k_sweep_xrow(SweepArgs<R, S> a) {
@@ -798,8 +804,9 @@ __global__ __launch_bounds__(256) void k_sweep_xrow(SweepArgs<R, S> a) {
   const int L = g.ix, rw = 256 / L;  // row length, rows per workgroup
   const int tid = threadIdx.x, lane = tid & 63;
   const int r = tid / L, p = tid - r * L;  // row in the workgroup, cell in the row
+  const int nbb = XROW_BAND / rw;  // workgroups per band
   const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int j = g.h + (t / g.iz) * rw + r, k = g.h + t % g.iz;
+  const int j = g.h + (t / (nbb * g.iz)) * XROW_BAND + (t % nbb) * rw + r, k = g.h + (t / nbb) % g.iz;
   const bool rowok = j < g.h + g.iy;
   const int i = g.h + p;
   const int rb = r * (L + SW - 1);  // the row's first position, cell h - HW

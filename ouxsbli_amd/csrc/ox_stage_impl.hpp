@@ -27,8 +27,10 @@ template <class S> Prim<S> prim_set(const Core& C, int set) {
 template <int ORDER, int VISC, int VO> dim3 grid_x(const Geom& g) {
   return dim3(cdiv(g.ix, xtile<ORDER, VISC, VO>()), cdiv(g.iy, XROWS), g.iz);
 }
-// k_sweep_xrow: 256 / ix rows per workgroup, a band of rows in one plane
-inline dim3 grid_xrow(const Geom& g) { return dim3((unsigned)(cdiv(g.iy, 256 / g.ix) * g.iz)); }
+// k_sweep_xrow: 256 / ix rows per workgroup, bands of XROW_BAND rows per plane
+inline dim3 grid_xrow(const Geom& g) {
+  return dim3((unsigned)(cdiv(g.iy, XROW_BAND) * g.iz * (XROW_BAND / (256 / g.ix))));
+}
 inline dim3 grid_y(const Geom& g, int lseg) {
   return dim3(cdiv(g.ix, LINE_X), cdiv(g.iz, LINE_Y), cdiv(g.iy, lseg));
 }

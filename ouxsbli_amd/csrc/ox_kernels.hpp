@@ -9,7 +9,8 @@
 //          formed from rho and p where a sweep reads a cell.
 //   Racc   residual accumulator in the state layout, type S.
 //
-// One stage = k_sweep_x (LDS row tiles; Ducros sensor; writes R = c_x*dE) -> k_sweep_line<DIR=1>
+// One stage = k_sweep_x (LDS row tiles; Ducros sensor; writes R = c_x*dE; on
+// periodic rows of 64, 128 or 256 cells k_sweep_xrow, whole rows in LDS) -> k_sweep_line<DIR=1>
 // (register sliding window along y; R += c_y*dF) -> k_sweep_line<DIR=2>
 // (sliding window along z; R += c_z*dG, then the RK combination, and the
 // primitives of the new state for the next stage).  k_prim (QJ -> primitives)

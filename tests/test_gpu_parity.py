@@ -129,6 +129,26 @@ def test_steps_bitwise_full_rows(case, ix):
     np.testing.assert_array_equal(Rg[sl], o.rhs()[sl])
 
 
+@pytest.mark.parametrize("ix", [128, 256])
+@pytest.mark.parametrize("case", [(6, "SLAU", "NS", 6, "hybrid", "HRSLAU2", 3),
+                                  (4, "KEEP", "NS", 4, "none", "HRSLAU2", 4),
+                                  (6, "SLAU", "NS", 6, "hybrid", "HRSLAU2", 3, "ME4Base")],
+                         ids=lambda c: "-".join(map(str, c)))
+def test_full_rows_several_bands(case, ix):
+    """The whole-row x sweep walks bands of 16 rows per plane: 40 rows make
+    two full bands and a partial one, so the band and plane arithmetic of the
+    workgroup index is exercised past the first band."""
+    spec = _spec(*case, n=(ix, 40, 12))
+    Q = _perturbed_tgv(spec, seed=6)
+    o = Oracle(spec)
+    o.set_state(Q)
+    o.step(2)
+    with Solver(spec) as s:
+        s.set_state(Q)
+        s.step(2)
+        np.testing.assert_array_equal(s.get_qj(), o.get_qj())
+
+
 def test_tgv64_keep4_100_steps():
     """The NS TGV at 64^3 (KEEP4, central-4 viscous, RK4, Ma 0.1): all five conserved fields after
     100 steps within 1e-12 relative of the oracle (expected: identical)."""
